@@ -1,0 +1,127 @@
+"""ctypes mirror of include/ddfgpu.h.
+
+The structs describe one gridding / degridding call in the layout of the reference argument
+lists of ``pyGridderWPol`` / ``pyDeGridderWPol`` (DDFacet/Gridder/GridderSmearPols.cc:76-97,
+212-233).  They carry raw addresses only, so the same description is used with device
+addresses (the HIP library) and with host addresses (the CPU oracle used by the tests).
+"""
+import ctypes as C
+
+DDF_ABI_VERSION = 1
+
+DDF_OK = 0
+DDF_EINVAL = -1
+DDF_ERUNTIME = -2
+DDF_EHIP = -3
+DDF_ENOMEM = -4
+
+P = C.c_void_p
+
+
+class ddf_cf(C.Structure):
+    _fields_ = [
+        ("data", P),
+        ("offset", P),
+        ("support", P),
+        ("nw", C.c_int32),
+        ("overs", C.c_int32),
+        ("ref_wave", C.c_double),
+        ("wmax", C.c_double),
+        ("max_support", C.c_int32),
+    ]
+
+
+class ddf_jones(C.Structure):
+    _fields_ = [
+        ("enabled", C.c_int32),
+        ("mode", C.c_int32),
+        ("interp_mode", C.c_int32),
+        ("kms", P),
+        ("kms_dims", C.c_int32 * 4),
+        ("kms_time", P),
+        ("kms_chan", P),
+        ("kms_dir", C.c_int32),
+        ("beam", P),
+        ("beam_dims", C.c_int32 * 4),
+        ("beam_time", P),
+        ("beam_chan", P),
+        ("beam_dir", C.c_int32),
+        ("a0", P),
+        ("a1", P),
+        ("coefs", P),
+        ("ncoefs", C.c_int32),
+        ("apply_amp", C.c_int32),
+        ("apply_phase", C.c_int32),
+        ("scale_jones", C.c_int32),
+        ("calib_error", C.c_double),
+        ("sum_jones", P),
+        ("sum_jones_chan", P),
+        ("reweight_snr", C.c_double),
+    ]
+
+
+class ddf_decorr(C.Structure):
+    _fields_ = [
+        ("enabled", C.c_int32),
+        ("uvw_dt", P),
+        ("dt", C.c_double),
+        ("dnu", C.c_double),
+        ("tsmear", C.c_int32),
+        ("fsmear", C.c_int32),
+        ("l0", C.c_double),
+        ("m0", C.c_double),
+    ]
+
+
+class ddf_vis(C.Structure):
+    _fields_ = [
+        ("vis", P),
+        ("uvw", P),
+        ("flags", P),
+        ("weights", P),
+        ("freqs", P),
+        ("chan_map", P),
+        ("nrow", C.c_int64),
+        ("nchan", C.c_int32),
+        ("ncorr", C.c_int32),
+        ("chan_equidistant", C.c_int32),
+        ("bda", P),
+        ("bda_len", C.c_int64),
+        ("nblocks", C.c_int64),
+        ("bda_start", P),
+        ("sparsification", P),
+        ("corr_codes", C.c_uint16 * 4),
+        ("ncorr_codes", C.c_int32),
+        ("stokes_codes", C.c_uint16 * 4),
+        ("nstokes", C.c_int32),
+    ]
+
+
+class ddf_facet(C.Structure):
+    _fields_ = [
+        ("grid", P),
+        ("sumwt", P),
+        ("cf", ddf_cf),
+        ("cu", C.c_double),
+        ("cv", C.c_double),
+        ("l0", C.c_double),
+        ("m0", C.c_double),
+        ("jones", ddf_jones),
+        ("decorr", ddf_decorr),
+    ]
+
+
+class ddf_grid_desc(C.Structure):
+    _fields_ = [
+        ("grid_f64", C.c_int32),
+        ("nband", C.c_int32),
+        ("npol", C.c_int32),
+        ("ny", C.c_int32),
+        ("nx", C.c_int32),
+        ("incr", C.c_double * 2),
+        ("dopsf", C.c_int32),
+    ]
+
+
+STOKES_CODES = {"undef": 0, "I": 1, "Q": 2, "U": 3, "V": 4, "RR": 5, "RL": 6, "LR": 7, "LL": 8,
+                "XX": 9, "XY": 10, "YX": 11, "YY": 12}

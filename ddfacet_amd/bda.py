@@ -1,0 +1,180 @@
+"""Baseline-dependent-averaging (BDA) block maps in the reference int32 format.
+
+Format (DDFacet/Data/ClassSmearMapping.py:83-135):
+    [NTot_lo, NTot_hi, size_0 .. size_{NTot-1}, rec_0, rec_1, ...]
+with rec_b = [chStart, chEnd, row, row, ...] and size_b = len(rec_b) = 2 + rows_b.  Per-baseline
+block lists are concatenated in lexicographic order of the string key "a0:a1"
+(ClassSmearMapping.py:113).
+
+``bda_map_mode1`` restates the default block builder ``GiveBlocksRowsListBL_old``
+(ClassSmearMapping.py:496-625, ``--Comp-BDAMode=1``) driven as ``computeSmearMappingInBackground``
+drives it (ClassSmearMapping.py:64-79, ClassVisServer.py:637-679).  The other builders are
+synthetic layouts used by the benchmarks: one block per (row, channel) and fixed
+time x frequency averaging factors per baseline.
+"""
+import numpy as np
+
+C_BDA = 3e8  # the block builder uses 3e8, not 299792458 (ClassSmearMapping.py:508)
+
+
+def pack(per_baseline):
+    """per_baseline: dict key "a0:a1" -> (list of sizes, flat int list).  Returns the int32 map."""
+    keys = sorted(per_baseline.keys())
+    sizes = [np.asarray(per_baseline[k][0], np.int64) for k in keys]
+    recs = [np.asarray(per_baseline[k][1], np.int64) for k in keys]
+    nblk = int(sum(s.size for s in sizes))
+    nrec = int(sum(r.size for r in recs))
+    out = np.zeros(2 + nblk + nrec, np.int32)
+    out[:2] = np.array([nblk & 0xFFFFFFFF, nblk >> 32], np.uint64).astype(np.uint32).view(np.int32)
+    if nblk:
+        out[2:2 + nblk] = np.concatenate(sizes)
+        out[2 + nblk:] = np.concatenate(recs)
+    return out
+
+
+def nblocks_of(bda):
+    return int(np.uint32(bda[0])) + (int(bda[1]) << 32)
+
+
+def iter_blocks(bda):
+    """Yield (chStart, chEnd, rows) of every block, in map order."""
+    nb = nblocks_of(bda)
+    sizes = bda[2:2 + nb]
+    pos = 2 + nb
+    for s in sizes:
+        rec = bda[pos:pos + s]
+        yield int(rec[0]), int(rec[1]), rec[2:]
+        pos += s
+
+
+def block_starts(bda):
+    """Word offset of each block record (what ddf_bda_index computes on the device)."""
+    nb = nblocks_of(bda)
+    sizes = bda[2:2 + nb].astype(np.int64)
+    starts = np.empty(nb, np.int64)
+    if nb:
+        starts[0] = 2 + nb
+        np.cumsum(sizes[:-1], out=starts[1:])
+        starts[1:] += 2 + nb
+    return starts
+
+
+def _baseline_rows(a0, a1):
+    """Row indices of each ordered antenna pair present in the data (a0 != a1)."""
+    out = {}
+    order = np.lexsort((a1, a0))
+    pairs = np.stack([a0[order], a1[order]], 1)
+    if order.size == 0:
+        return out
+    brk = np.nonzero(np.any(pairs[1:] != pairs[:-1], axis=1))[0] + 1
+    for seg in np.split(np.arange(order.size), brk):
+        p, q = int(pairs[seg[0], 0]), int(pairs[seg[0], 1])
+        if p == q:
+            continue
+        out["%d:%d" % (p, q)] = np.sort(order[seg])
+    return out
+
+
+def _blocks_bl_old(ind, uvw, dfreq, freqs, dphi, lrad, chan_map):
+    """One baseline of GiveBlocksRowsListBL_old (ClassSmearMapping.py:496-625)."""
+    nrows = ind.size
+    nchan = freqs.size
+    nu0 = np.max(freqs)
+    u, v, w = uvw[ind, :].T
+    du, dv, dw = u[1:] - u[:-1], v[1:] - v[:-1], w[1:] - w[:-1]
+    if du.size < 1:
+        du, dv, dw = np.array([1e-30]), np.array([1e-30]), np.array([1e-30])
+    du = np.concatenate((du, [du[-1]]))
+    dv = np.concatenate((dv, [dv[-1]]))
+    dw = np.concatenate((dw, [dw[-1]]))
+    duv_cut = C_BDA * dphi / (np.pi * lrad * nu0)
+    uvd = np.maximum(np.sqrt(u ** 2 + v ** 2 + w ** 2), 1e-10)
+    nch_block = ((C_BDA / np.pi) * dphi / (uvd * lrad)) / dfreq
+    step = np.sqrt(du ** 2 + dv ** 2 + dw ** 2)
+    sizes, flat = [], []
+    cur = []
+    duvtot = 0.0
+    nchb_max = 1e3
+    for i in range(nrows):
+        cur.append(int(ind[i]))
+        if nch_block[i] < nchb_max:
+            nchb_max = nch_block[i]
+        duvtot += step[i]
+        if (duvtot > duv_cut) or (i == nrows - 1):
+            nchb_max = np.max([nchb_max, 1])
+            ch = np.arange(0, nchan, nchb_max).tolist()
+            if nchan not in ch:
+                ch.append(nchan)
+            chb = np.int32(np.linspace(0, nchan, len(ch)))
+            rampe = np.zeros(nchan, np.int32)
+            for k in range(chb.size - 1):
+                rampe[chb[k]:chb[k + 1]] = k
+            cuts = []
+            prev = (-1, -1)
+            for c in range(nchan):
+                cur_key = (rampe[c], chan_map[c])
+                if cur_key != prev:
+                    cuts.append(c)
+                prev = cur_key
+            if chb[-1] not in cuts:
+                cuts.append(chb[-1])
+            cuts = np.array(cuts, np.int32)
+            for k in range(cuts.size - 1):
+                rec = [int(cuts[k]), int(cuts[k + 1])] + cur
+                sizes.append(len(rec))
+                flat.extend(rec)
+            nchb_max = 1e3
+            cur = []
+            duvtot = 0.0
+    return sizes, flat
+
+
+def bda_map_mode1(uvw, a0, a1, freqs, dfreq, decorr, cell_rad, facet_npix, chan_map):
+    """BDA.Grid / BDA.Degrid map of the default builder.
+
+    decorr: --Comp-GridDecorr (or DegridDecorr); the builder receives (1 - decorr) and uses
+    dPhi = sqrt(6 (1 - (1 - decorr))) (ClassSmearMapping.py:66, ClassVisServer.py:664-666).
+    cell_rad, facet_npix: the FoV radius is cell * npix * sqrt(2)/2 (ClassVisServer.py:661-663,
+    --Comp-GridFoV=Facet uses the non-padded facet size).
+    """
+    decorr_in = 1.0 - decorr
+    dphi = np.sqrt(6. * (1. - decorr_in))
+    fov_deg = cell_rad * facet_npix * (np.sqrt(2.) / 2.) * 180. / np.pi
+    lrad = fov_deg * np.pi / 180
+    chan_map = np.asarray(chan_map)
+    per_bl = {}
+    for key, ind in _baseline_rows(np.asarray(a0), np.asarray(a1)).items():
+        per_bl[key] = _blocks_bl_old(ind, uvw, dfreq, np.asarray(freqs), dphi, lrad, chan_map)
+    return pack(per_bl)
+
+
+def bda_map_per_visibility(a0, a1, nchan):
+    """One block per (row, channel): no averaging."""
+    per_bl = {}
+    for key, ind in _baseline_rows(np.asarray(a0), np.asarray(a1)).items():
+        n = ind.size
+        rec = np.empty((n, nchan, 3), np.int64)
+        rec[:, :, 0] = np.arange(nchan)[None, :]
+        rec[:, :, 1] = np.arange(nchan)[None, :] + 1
+        rec[:, :, 2] = ind[:, None]
+        per_bl[key] = (np.full(n * nchan, 3, np.int64), rec.ravel())
+    return pack(per_bl)
+
+
+def bda_map_factors(a0, a1, nchan, factor_of_baseline, chan_map=None):
+    """Fixed averaging per baseline: factor_of_baseline(key) -> (k_time, k_freq).  Channel blocks
+    are also cut where chan_map changes, so a block never straddles two grid bands."""
+    chan_map = np.zeros(nchan, np.int64) if chan_map is None else np.asarray(chan_map)
+    per_bl = {}
+    for key, ind in _baseline_rows(np.asarray(a0), np.asarray(a1)).items():
+        kt, kf = factor_of_baseline(key)
+        cuts = sorted(set(list(range(0, nchan, max(1, kf))) + [nchan] +
+                          [int(c) for c in np.nonzero(np.diff(chan_map))[0] + 1]))
+        sizes, flat = [], []
+        for t0 in range(0, ind.size, max(1, kt)):
+            rows = ind[t0:t0 + kt].tolist()
+            for c0, c1 in zip(cuts[:-1], cuts[1:]):
+                sizes.append(2 + len(rows))
+                flat.extend([c0, c1] + rows)
+        per_bl[key] = (sizes, flat)
+    return pack(per_bl)

@@ -1,0 +1,161 @@
+/*
+ * ddfgpu.h -- C ABI of the MI355X (gfx950) facet gridder / degridder.
+ *
+ * This library replaces the native module DDFacet.cbuild.Gridder._pyGridderSmearPols3x
+ * (DDFacet/Gridder/GridderSmearPols.cc:349-370).  The Python shim
+ * ddfacet_amd/_pyGridderSmearPols.py exports the reference names with the reference argument
+ * lists and marshals them into the structs below; any other host (C, C++, Go over cgo, ...)
+ * can call these entry points directly.
+ *
+ * Conventions
+ *   - Every pointer inside an args struct is a DEVICE pointer (hipMalloc'd memory or a torch
+ *     tensor's data_ptr()) unless the field comment says "host".
+ *   - Entry points are asynchronous on `stream` (a hipStream_t passed as void*, NULL = the null
+ *     stream) and return DDF_OK or a negative error code; ddf_last_error() returns the message of
+ *     the last failure on the calling thread.  Argument errors are detected before any launch.
+ *   - complex64 data are interleaved (re, im) float pairs; complex128 are (re, im) double pairs.
+ *   - Correlation / Stokes codes are the casacore Stokes enum used by DDFacet
+ *     (DDFacet/Data/ClassStokes.py:39-41): I=1 Q=2 U=3 V=4 RR=5 RL=6 LR=7 LL=8 XX=9 XY=10 YX=11 YY=12.
+ */
+#ifndef DDFGPU_H
+#define DDFGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DDF_ABI_VERSION 1
+
+enum ddf_status {
+  DDF_OK = 0,
+  DDF_EINVAL = -1,    /* invalid argument (reference: std::invalid_argument -> ValueError)   */
+  DDF_ERUNTIME = -2,  /* runtime failure  (reference: std::runtime_error   -> RuntimeError)  */
+  DDF_EHIP = -3,      /* HIP / hipFFT API failure                                            */
+  DDF_ENOMEM = -4     /* device allocation failure                                           */
+};
+
+/* Per-facet w-projection convolution functions, the content of ClassWTermModified.Wplanes and
+ * .WplanesConj (DDFacet/Imager/ModCF.py:560-593).  Plane p (0 <= p < nw) is Wplanes[p], plane
+ * nw + p is WplanesConj[p]; both are complex64 [(S*overs)^2] in the reorganised layout
+ * [overs][overs][S][S] (ModCF.py:364-372), S = support[p] (odd).                               */
+typedef struct ddf_cf {
+  const float*   data;      /* complex64, all planes packed                                     */
+  const int64_t* offset;    /* [2*nw] complex-element offset of each plane inside data          */
+  const int32_t* support;   /* [nw] support S of plane p (shared by Wplanes[p], WplanesConj[p]) */
+  int32_t nw;               /* WInfos[2] = len(Wplanes)                                         */
+  int32_t overs;            /* WInfos[3] = OverS                                                */
+  double  ref_wave;         /* WInfos[0] = c / max(freq)                                        */
+  double  wmax;             /* WInfos[1]                                                        */
+  int32_t max_support;      /* host: max over support[] (sizes launch resources)                */
+} ddf_cf;
+
+/* The 20-entry LJones list (DDFacet/Imager/ClassDDEGridMachine.py:571-679, 759-776) as read by
+ * DDEs::JonesServer (DDFacet/Gridder/JonesServer.cc:97-191).  enabled=0 <=> LJones == [].      */
+typedef struct ddf_jones {
+  int32_t enabled;
+  int32_t mode;             /* LJones[9][1]: 1 = per visibility, 2 = per BDA block              */
+  int32_t interp_mode;      /* LJones[9][0]: 0 = nearest, 1 = "Krigging"                        */
+  const float*   kms;       /* LJones[0] complex64 [nt][nd][na][nch][2][2] or NULL              */
+  int32_t kms_dims[4];      /* host: nt, nd, na, nch                                            */
+  const int32_t* kms_time;  /* LJones[1]  [nrow]                                                */
+  const int32_t* kms_chan;  /* LJones[10] [nchan]                                               */
+  int32_t kms_dir;          /* LJones[6][0]                                                     */
+  const float*   beam;      /* LJones[2] complex64 [nt][nd][na][nch][2][2] or NULL              */
+  int32_t beam_dims[4];
+  const int32_t* beam_time; /* LJones[3]  [nrow]                                                */
+  const int32_t* beam_chan; /* LJones[11] [nchan]                                               */
+  int32_t beam_dir;         /* LJones[8][0]                                                     */
+  const int32_t* a0;        /* LJones[4] [nrow]                                                 */
+  const int32_t* a1;        /* LJones[5] [nrow]                                                 */
+  const float*   coefs;     /* LJones[7] [ncoefs] (interp_mode 1 only) or NULL                  */
+  int32_t ncoefs;
+  int32_t apply_amp;        /* LJones[13]                                                       */
+  int32_t apply_phase;      /* LJones[14]                                                       */
+  int32_t scale_jones;      /* LJones[15]                                                       */
+  double  calib_error;      /* LJones[16]                                                       */
+  double* sum_jones;        /* LJones[17] float64 [2][nband], accumulated (gridding only)       */
+  double* sum_jones_chan;   /* LJones[18] float64 [2][nchan], accumulated (gridding only)       */
+  double  reweight_snr;     /* LJones[19] (unused: every reference call site passes
+                               EstimateWeight=false, gridder.h:236,294 degridder.h:220,245)    */
+} ddf_jones;
+
+/* The 7-entry LSmearing list (ClassDDEGridMachine.py:545-569, DecorrelationHelper.cc:25-41).   */
+typedef struct ddf_decorr {
+  int32_t enabled;          /* len(LSmearing) != 0                                              */
+  const double* uvw_dt;     /* [nrow][3]                                                        */
+  double  dt, dnu;
+  int32_t tsmear, fsmear;
+  double  l0, m0;
+} ddf_decorr;
+
+/* Visibility-side arguments shared by gridding and degridding.                                  */
+typedef struct ddf_vis {
+  float*         vis;       /* complex64 [nrow][nchan][ncorr] (read by grid, vis -= model by degrid) */
+  const double*  uvw;       /* [nrow][3]                                                        */
+  const uint8_t* flags;     /* bool [nrow][nchan][ncorr]                                        */
+  const float*   weights;   /* [nrow][nchan] (gridding only)                                    */
+  const double*  freqs;     /* [nchan]                                                          */
+  const int32_t* chan_map;  /* [nchan] visibility channel -> grid band                          */
+  int64_t nrow;
+  int32_t nchan, ncorr;
+  int32_t chan_equidistant; /* LOptimisation[1]                                                 */
+  /* BDA block map, int32 [2 + nblocks + sum(2 + rows_b)] (DDFacet/Data/ClassSmearMapping.py:83-135) */
+  const int32_t* bda;
+  int64_t bda_len;          /* host: number of int32 words in bda                               */
+  int64_t nblocks;          /* host: bda[0] + (bda[1] << 32)                                    */
+  const int64_t* bda_start; /* optional [nblocks] word offset of block b's (chStart, chEnd, rows...)
+                               record inside bda (from ddf_bda_index); NULL = computed per call */
+  const uint8_t* sparsification; /* bool [nblocks] or NULL (gridding only)                      */
+  uint16_t corr_codes[4];   /* DataCorrFormat                                                   */
+  int32_t  ncorr_codes;
+  uint16_t stokes_codes[4]; /* ExpectedOutStokes                                                */
+  int32_t  nstokes;
+} ddf_vis;
+
+/* One facet: its grid and its geometry.                                                         */
+typedef struct ddf_facet {
+  void*   grid;             /* complex64 (or complex128 if grid_f64) [nband][npol][ny][nx]      */
+  double* sumwt;            /* float64 [nband][npol], accumulated (gridding only)               */
+  ddf_cf  cf;
+  double  cu, cv, l0, m0;   /* FacetInfos[0:4] (ClassDDEGridMachine.py:748-749)                  */
+  ddf_jones jones;
+  ddf_decorr decorr;
+} ddf_facet;
+
+typedef struct ddf_grid_desc {
+  int32_t grid_f64;         /* 0: complex64 grid (pyGridderWPol32), 1: complex128 (pyGridderWPol64) */
+  int32_t nband, npol, ny, nx;
+  double  incr[2];          /* increment: (-cell, +cell) radians                                */
+  int32_t dopsf;
+} ddf_grid_desc;
+
+/* ---- library ------------------------------------------------------------------------------- */
+int         ddf_version(void);               /* DDF_ABI_VERSION                                  */
+const char* ddf_last_error(void);
+int         ddf_init(int device);            /* optional: select device, create workspace        */
+
+/* Word offsets of every BDA block record, for reuse across calls (major cycles).
+ * bda: device int32 map, out: device int64 [nblocks].                                           */
+int ddf_bda_index(const int32_t* bda, int64_t bda_len, int64_t nblocks, int64_t* out, void* stream);
+
+/* ---- plain single-facet entry points ---------------------------------------------------------
+ * ddf_grid replaces pyGridderWPol<gridtype> (DDFacet/Gridder/GridderSmearPols.cc:76-209 ->
+ * gridder::gridder, gridder.h:78-377): grid, sumwt, sum_jones, sum_jones_chan are accumulated.
+ * ddf_degrid replaces pyDeGridderWPol (GridderSmearPols.cc:212-348 -> degridder::degridder,
+ * degridder.h:55-266): vis -= model.  The caller passes the CF lists as the reference call sites
+ * do: put() passes (Wplanes, WplanesConj), get() passes (WplanesConj, Wplanes)
+ * (ClassDDEGridMachine.py:801-802, 1091-1092); here plane p < nw is the list given first.       */
+int ddf_grid(const ddf_grid_desc* g, const ddf_vis* v, const ddf_facet* f, void* stream);
+int ddf_degrid(const ddf_grid_desc* g, const ddf_vis* v, const ddf_facet* f, void* stream);
+
+/* Weights scatter-add, replaces pyAccumulateWeightsOntoGrid[NoSem] (GridderSmearPols.cc:32-74):
+ * grid[index[i]] += weights[i] for weights[i] != 0.  grid float64, weights float32, index int64. */
+int ddf_accumulate_weights(double* grid, int64_t grid_len, const float* weights,
+                           const int64_t* index, int64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DDFGPU_H */
