@@ -1,0 +1,265 @@
+// Device-side building blocks of the gfx950 facet gridder / degridder.
+//
+// Complex and 2x2 Jones algebra, 64-lane wavefront reductions, the per-visibility phasor, the
+// Jones-table lookup of DDEs::JonesServer (DDFacet/Gridder/JonesServer.cc:25-298), the
+// decorrelation factor (DecorrelationHelper.cc:42-63) and the BDA-block placement arithmetic
+// (gridder.h:319-351, degridder.h:170-200) evaluated without FMA contraction so that the integer
+// maps it produces (w-plane, grid cell, oversampling offset) are those of the reference.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ddf {
+
+constexpr double kC = 299792458.0;
+constexpr double kPi = 3.141592653589793238462643383279502884197;
+constexpr int kWave = 64;
+
+// ---- complex -------------------------------------------------------------------------------
+struct cd {
+  double re, im;
+};
+__device__ __forceinline__ cd mk(double r, double i) { return cd{r, i}; }
+__device__ __forceinline__ cd operator+(cd a, cd b) { return cd{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cd operator-(cd a, cd b) { return cd{a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cd operator*(cd a, cd b) {
+  return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cd operator*(cd a, double s) { return cd{a.re * s, a.im * s}; }
+__device__ __forceinline__ cd conj(cd a) { return cd{a.re, -a.im}; }
+__device__ __forceinline__ double cabs(cd a) { return hypot(a.re, a.im); }
+
+struct M2 {
+  cd a[4];
+};
+__device__ __forceinline__ M2 unity() { return M2{{cd{1, 0}, cd{0, 0}, cd{0, 0}, cd{1, 0}}}; }
+__device__ __forceinline__ M2 mul(const M2& x, const M2& y) {
+  return M2{{x.a[0] * y.a[0] + x.a[1] * y.a[2], x.a[0] * y.a[1] + x.a[1] * y.a[3],
+             x.a[2] * y.a[0] + x.a[3] * y.a[2], x.a[2] * y.a[1] + x.a[3] * y.a[3]}};
+}
+__device__ __forceinline__ M2 herm(const M2& x) {
+  return M2{{conj(x.a[0]), conj(x.a[2]), conj(x.a[1]), conj(x.a[3])}};
+}
+
+// ---- wavefront reductions (all lanes receive the total) ------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// ---- phasor ---------------------------------------------------------------------------------
+// exp(i*phase) with the phase formed in FP64 and reduced to [-pi, pi] in FP64 (2*pi split in a
+// head and a tail), then evaluated by the FP32 sincosf: the error is ~1e-7 for any |phase|,
+// where an FP32 phase of 1e4 rad would already carry ~1e-3 rad.
+__device__ __forceinline__ cd phasor(double phase) {
+  const double inv2pi = 0.15915494309189533577;
+  const double twopi_hi = 6.28318530717958623200;
+  const double twopi_lo = 2.44929359829470635445e-16;
+  double k = rint(phase * inv2pi);
+  double r = fma(-k, twopi_hi, phase);
+  r = fma(-k, twopi_lo, r);
+  float s, c;
+  sincosf(float(r), &s, &c);
+  return cd{double(c), double(s)};
+}
+
+// ---- Jones tables (JonesServer.cc) -----------------------------------------------------------
+struct JonesP {
+  int enabled, mode, interp;
+  const float2* kms;
+  int kdims[4];
+  const int32_t *ktime, *kchan;
+  int kdir;
+  const float2* beam;
+  int bdims[4];
+  const int32_t *btime, *bchan;
+  int bdir;
+  const int32_t *a0, *a1;
+  const float* coefs;
+  int ncoefs;
+  int apply_amp, apply_phase, scale;
+  double calib_error;
+  double* sum_jones;
+  double* sum_jones_chan;
+};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// GiveJones (JonesServer.cc:54-95); indices clamped to the table so a bad map cannot fault.
+__device__ inline M2 give_jones(const float2* tab, const int* dims, int it, int ant, int dir, int ich,
+                                int interp, const float* coefs, int ncoefs) {
+  it = clampi(it, 0, dims[0] - 1);
+  dir = clampi(dir, 0, dims[1] - 1);
+  ant = clampi(ant, 0, dims[2] - 1);
+  ich = clampi(ich, 0, dims[3] - 1);
+  size_t off = ((size_t(it) * dims[1] + dir) * size_t(dims[2]) + ant) * size_t(dims[3]) + ich;
+  const float2* p = tab + off * 4;
+  M2 out;
+  if (interp == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) out.a[q] = cd{p[q].x, p[q].y};
+    return out;
+  }
+  // Interpolation mode 1 reads the same direction for every coefficient (JonesServer.cc:71-93):
+  // the result is (sum of non-zero coefs)^2 times the Jones of `dir`, accumulated in the same
+  // order as the reference.
+  double jabs[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out.a[q] = cd{0, 0};
+  for (int d = 0; d < ncoefs; ++d) {
+    double c = coefs[d];
+    if (c == 0) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      cd val{p[q].x, p[q].y};
+      double A = cabs(val);
+      out.a[q] = out.a[q] + val * (c / A);
+      jabs[q] += c * A;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out.a[q] = out.a[q] * jabs[q];
+  return out;
+}
+
+// NormJones (JonesServer.cc:25-52), applied to the DD (killMS) Jones only.
+__device__ inline void norm_jones(const JonesP& jp, M2& J, const double* uvw, double wl_mean) {
+  if (!jp.apply_amp) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double a = cabs(J.a[q]);
+      if (a != 0.) J.a[q] = cd{J.a[q].re / a, J.a[q].im / a};
+    }
+  }
+  if (!jp.apply_phase) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) J.a[q] = cd{cabs(J.a[q]), 0.};
+  }
+  if (jp.scale) {
+    double R2 = (uvw[0] * uvw[0] + uvw[1] * uvw[1]) / (wl_mean * wl_mean);
+    double al = exp(-2. * kPi * jp.calib_error * jp.calib_error * R2);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double a = cabs(J.a[q]);
+      if (a != 0.) J.a[q] = J.a[q] * ((1. - al) / a + al);
+    }
+  }
+}
+
+struct JonesPair {
+  M2 J0, J1;
+  double BB;
+};
+
+// updateJones (JonesServer.cc:193-298) evaluated afresh for (row, chan): J = JkMS * JBeam per
+// antenna, BB = ((|J0_00||J1_00| + |J0_11||J1_11|)/2)^2.  `uvw` feeds ScaleJones only.
+__device__ inline JonesPair jones_at(const JonesP& jp, int64_t row, int chan, const double* uvw,
+                                     double wl_mean) {
+  int a0 = jp.a0[row], a1 = jp.a1[row];
+  M2 J0 = unity(), J1 = unity();
+  if (jp.beam) {
+    int t = jp.btime[row], c = jp.bchan[chan];
+    J0 = give_jones(jp.beam, jp.bdims, t, a0, jp.bdir, c, jp.interp, jp.coefs, jp.ncoefs);
+    J1 = give_jones(jp.beam, jp.bdims, t, a1, jp.bdir, c, jp.interp, jp.coefs, jp.ncoefs);
+  }
+  if (jp.kms) {
+    int t = jp.ktime[row], c = jp.kchan[chan];
+    M2 K0 = give_jones(jp.kms, jp.kdims, t, a0, jp.kdir, c, jp.interp, jp.coefs, jp.ncoefs);
+    M2 K1 = give_jones(jp.kms, jp.kdims, t, a1, jp.kdir, c, jp.interp, jp.coefs, jp.ncoefs);
+    norm_jones(jp, K0, uvw, wl_mean);
+    norm_jones(jp, K1, uvw, wl_mean);
+    J0 = mul(K0, J0);
+    J1 = mul(K1, J1);
+  }
+  JonesPair r;
+  r.J0 = J0;
+  r.J1 = J1;
+  double bb = (cabs(J0.a[0]) * cabs(J1.a[0]) + cabs(J0.a[3]) * cabs(J1.a[3])) / 2.;
+  r.BB = bb * bb;
+  return r;
+}
+
+// ---- decorrelation (DecorrelationHelper.cc:42-63) -------------------------------------------
+struct DecorrP {
+  int enabled, tsmear, fsmear;
+  const double* uvw_dt;
+  double dt, dnu, l0, m0;
+};
+
+__device__ inline double decorr_factor(const DecorrP& d, const double* uvw, double nu, int64_t idx) {
+  if (!d.enabled) return 1.;
+  double n0 = sqrt(1. - d.l0 * d.l0 - d.m0 * d.m0) - 1.;
+  double f = 1.;
+  if (d.fsmear) {
+    const double* q = uvw + 3 * idx;
+    double ph = q[0] * d.l0 + q[1] * d.m0 + q[2] * n0;
+    double phi = kPi * d.dnu / kC * ph;
+    f *= (phi == 0.) ? 1.0 : sin(phi) / phi;
+  }
+  if (d.tsmear) {
+    const double* q = d.uvw_dt + 3 * idx;
+    double dph = (q[0] * d.l0 + q[1] * d.m0 + q[2] * n0) * d.dt;
+    double phi = kPi * nu / kC * dph;
+    f *= (phi == 0.) ? 1.0 : sin(phi) / phi;
+  }
+  return f;
+}
+
+// ---- CF tables and BDA-block placement -------------------------------------------------------
+struct CFP {
+  const float2* data;
+  const int64_t* offset;  // [2*nw]
+  const int32_t* support; // [nw]
+  int nw, overs;
+  double ref_wave, wmax;
+};
+
+struct Place {
+  int locx, locy, sup, S;
+  int64_t cfoff;  // complex-element offset of the oversampled sub-kernel
+};
+
+// w-plane, grid cell and oversampling offset of a block (gridder.h:321-351): evaluated in the
+// reference operation order with contraction off.  Returns false when the block is dropped
+// (iw >= Nw or support outside the grid), as the reference does.
+__device__ inline bool place_block(const CFP& cf, int nx, int ny, double incr0, double incr1,
+                                   double um, double vm, double wm, double recip_wvl, Place* pl) {
+#pragma clang fp contract(off)
+  double fiw = (double(cf.nw) - 1) * fabs(wm) * (cf.ref_wave * recip_wvl) / cf.wmax;
+  if (!(fiw >= 0.0 && fiw < 2147483647.0)) return false;
+  int iw = int(rint(fiw));
+  if (iw >= cf.nw) return false;
+  int plane = (wm > 0) ? iw : cf.nw + iw;
+  int S = cf.support[iw];
+  int O = cf.overs;
+  int sup = (S - 1) / 2;
+  double posx = double(nx) * incr0 * um * recip_wvl + double(nx / 2);
+  double posy = double(nx) * incr1 * vm * recip_wvl + double(ny / 2);
+  if (!(fabs(posx) < 1e9 && fabs(posy) < 1e9)) return false;
+  int locx = int(rint(posx)), locy = int(rint(posy));
+  if (locx - sup < 0 || locx + sup >= nx || locy - sup < 0 || locy + sup >= ny) return false;
+  int offx = int(rint((double(locx) - posx) * double(O))) + (S * O - 1) / 2;
+  int offy = int(rint((double(locy) - posy) * double(O))) + (S * O - 1) / 2;
+  int io = clampi(offy - sup * O, 0, O - 1);
+  int jo = clampi(offx - sup * O, 0, O - 1);
+  pl->locx = locx;
+  pl->locy = locy;
+  pl->sup = sup;
+  pl->S = S;
+  pl->cfoff = cf.offset[plane] + int64_t(io * O + jo) * S * S;
+  return true;
+}
+
+// Block-sum term U + W*Cu of the reference (gridder.h:272-274), no contraction.
+__device__ __forceinline__ double shifted(double u, double w, double c) {
+#pragma clang fp contract(off)
+  return u + w * c;
+}
+
+}  // namespace ddf

@@ -1,0 +1,915 @@
+// gfx950 facet gridder / degridder: plain single-facet kernels and the C ABI of include/ddfgpu.h.
+//
+// Work decomposition (one facet per launch, the "plain" path):
+//   * one 64-lane wavefront per BDA block (grid-stride over blocks);
+//   * the block's (row, channel) visibilities are spread over the lanes, channel fastest, so a
+//     wavefront reads a row's consecutive channels as one coalesced segment;
+//   * block sums (visibility, uvw means, weights) are reduced across the wavefront by butterfly
+//     shuffles; the w-plane / cell / oversampling decision is then lane-uniform;
+//   * gridding scatters the S x S x npol stamp with one lane per cell through global float
+//     atomics; degridding gathers the same stamp, reduces it, and writes every (row, channel)
+//     of the block back once (blocks of a degrid map are disjoint).
+//   * sumwt / SumJones / SumJonesChan go through per-workgroup LDS accumulators flushed once per
+//     workgroup with global FP64 atomics.
+// Follows gridder::gridder (DDFacet/Gridder/gridder.h:78-377) and degridder::degridder
+// (degridder.h:55-266); argument dispatch follows GridderSmearPols.cc:76-348.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ddfgpu.h"
+#include "ddf_device.h"
+
+using namespace ddf;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) return fail(DDF_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---- correlation <-> Stokes maps (Stokes.h), as 4x4 complex matrices -----------------------
+struct StokesMap {
+  int read;        // 4, 2, 1: Read_4Corr / Read_2Corr_Pad / Read_1Corr_Pad
+  int nstokes;
+  double m[4][4][2];
+};
+
+bool same(const uint16_t* c, int n, std::initializer_list<int> w) {
+  if (n != int(w.size())) return false;
+  int i = 0;
+  for (int x : w)
+    if (c[i++] != x) return false;
+  return true;
+}
+
+std::string stokes_string(const ddf_vis* v) {
+  std::string s;
+  for (int i = 0; i < v->nstokes; ++i) s += (v->stokes_codes[i] >= 1 && v->stokes_codes[i] <= 4) ? "?IQUV"[v->stokes_codes[i]] : '!';
+  return s;
+}
+
+void set(double (&m)[4][4][2], int r, int c, double re, double im) {
+  m[r][c][0] = re;
+  m[r][c][1] = im;
+}
+
+// Grid side, GridderSmearPols.cc:128-206 with the Stokes.h:18-34 formulas.
+int grid_map(const ddf_vis* v, StokesMap* sm) {
+  const uint16_t* c = v->corr_codes;
+  int nc = v->ncorr_codes;
+  for (int i = 0; i < nc; ++i)
+    if (!((c[i] >= 5 && c[i] <= 12) || c[i] == 1))
+      return fail(DDF_EINVAL, "Only accepts I,RR,RL,LR,LL,XX,XY,YX,YY as correlation input type");
+  for (int i = 0; i < v->nstokes; ++i)
+    if (v->stokes_codes[i] < 1 || v->stokes_codes[i] > 4)
+      return fail(DDF_EINVAL, "Only accepts I,Q,U,V as polarization output type");
+  std::memset(sm, 0, sizeof(*sm));
+  std::string want = stokes_string(v), eff = want;
+  int kind;  // 0 circular 4-corr, 1 linear 4-corr, 2 linear 2-corr, 3 circular 2-corr, 4 Stokes I
+  std::vector<std::string> ok;
+  if (same(c, nc, {9, 10, 11, 12})) { kind = 1; sm->read = 4; ok = {"I", "Q", "U", "V", "IQ", "IU", "IV", "QU", "IQU", "IQUV"}; }
+  else if (same(c, nc, {5, 6, 7, 8})) { kind = 0; sm->read = 4; ok = {"I", "IQ", "IU", "IV", "QU", "IQU", "IQUV"}; }
+  else if (same(c, nc, {9, 12})) { kind = 2; sm->read = 2; ok = {"I", "IQ"}; }
+  else if (same(c, nc, {5, 8})) { kind = 3; sm->read = 2; ok = {"I", "IV"}; }
+  else if (same(c, nc, {1})) { kind = 4; sm->read = 1; ok = {"I"}; }
+  else return fail(DDF_EINVAL, "Cannot convert input correlations to desired output Stokes parameters.");
+  bool found = false;
+  for (auto& o : ok) found |= (o == want);
+  if (!found) return fail(DDF_EINVAL, "Cannot convert input correlations to desired output Stokes parameters.");
+  if (kind == 1 && want == "IU") eff = "IQ";  // the {I,U} request runs the IQ policy (GridderSmearPols.cc:142-143)
+  sm->nstokes = v->nstokes;
+  for (int k = 0; k < sm->nstokes; ++k) {
+    char s = eff[k];
+    auto& m = sm->m;
+    if (kind == 4) { set(m, k, 0, 1, 0); continue; }
+    if (s == 'I') { set(m, k, 0, .5, 0); set(m, k, 3, .5, 0); continue; }
+    bool lin = (kind == 1 || kind == 2);
+    if ((lin && s == 'Q') || (!lin && s == 'V')) { set(m, k, 0, .5, 0); set(m, k, 3, -.5, 0); }
+    else if ((lin && s == 'U') || (!lin && s == 'Q')) { set(m, k, 1, .5, 0); set(m, k, 2, .5, 0); }
+    else { set(m, k, 1, 0, -.5); set(m, k, 2, 0, .5); }  // V (linear) / U (circular): -(c1-c2)*0.5i
+  }
+  return DDF_OK;
+}
+
+// Degrid side, GridderSmearPols.cc:261-342 with the Stokes.h:90-192 formulas.
+int degrid_map(const ddf_vis* v, StokesMap* sm) {
+  const uint16_t* c = v->corr_codes;
+  int nc = v->ncorr_codes;
+  for (int i = 0; i < nc; ++i)
+    if (!((c[i] >= 5 && c[i] <= 12) || c[i] == 1))
+      return fail(DDF_EINVAL, "Only accepts I,RR,RL,LR,LL,XX,XY,YX,YY as correlation output types");
+  std::memset(sm, 0, sizeof(*sm));
+  std::string w = stokes_string(v);
+  bool X4 = same(c, nc, {9, 10, 11, 12}), R4 = same(c, nc, {5, 6, 7, 8});
+  bool X2 = same(c, nc, {9, 12}), R2 = same(c, nc, {5, 8}), S1 = same(c, nc, {1});
+  auto& m = sm->m;
+  // rows: correlation slot (2x2 flattened), columns: Stokes index
+  if (w == "I" && (X4 || X2 || R4 || R2 || S1)) { set(m, 0, 0, 1, 0); set(m, 3, 0, 1, 0); }
+  else if (w == "Q" && X4) { set(m, 0, 0, 1, 0); set(m, 3, 0, -1, 0); }
+  else if (w == "U" && X4) { set(m, 1, 0, 1, 0); set(m, 2, 0, 1, 0); }
+  else if (w == "V" && X4) { set(m, 1, 0, 0, 1); set(m, 2, 0, 0, -1); }
+  else if (w == "IQUV" && X4) { set(m, 0, 0, 1, 0); set(m, 0, 1, 1, 0); set(m, 1, 2, 1, 0); set(m, 1, 3, 0, 1);
+                                set(m, 2, 2, 1, 0); set(m, 2, 3, 0, -1); set(m, 3, 0, 1, 0); set(m, 3, 1, -1, 0); }
+  else if (w == "IQUV" && R4) { set(m, 0, 0, 1, 0); set(m, 0, 3, 1, 0); set(m, 1, 1, 1, 0); set(m, 1, 2, 0, 1);
+                                set(m, 2, 1, 1, 0); set(m, 2, 2, 0, -1); set(m, 3, 0, 1, 0); set(m, 3, 3, -1, 0); }
+  else if (w == "IQ" && (X4 || X2)) { set(m, 0, 0, 1, 0); set(m, 0, 1, 1, 0); set(m, 3, 0, 1, 0); set(m, 3, 1, -1, 0); }
+  else if (w == "IQ" && R4) { set(m, 0, 0, 1, 0); set(m, 1, 1, 1, 0); set(m, 2, 1, 1, 0); set(m, 3, 0, 1, 0); }
+  else if (w == "IU" && X4) { set(m, 0, 0, 1, 0); set(m, 1, 1, 1, 0); set(m, 2, 1, 1, 0); set(m, 3, 0, 1, 0); }
+  else if (w == "IU" && R4) { set(m, 0, 0, 1, 0); set(m, 1, 1, 0, 1); set(m, 2, 1, 0, -1); set(m, 3, 0, 1, 0); }
+  else if (w == "IV" && (R4 || R2)) { set(m, 0, 0, 1, 0); set(m, 0, 1, 1, 0); set(m, 3, 0, 1, 0); set(m, 3, 1, -1, 0); }
+  else if (w == "IV" && X4) { set(m, 0, 0, 1, 0); set(m, 1, 1, 0, 1); set(m, 2, 1, 0, -1); set(m, 3, 0, 1, 0); }
+  else if (w == "QU" && X4) { set(m, 0, 0, 1, 0); set(m, 1, 1, 1, 0); set(m, 2, 1, 1, 0); set(m, 3, 0, -1, 0); }
+  else if (w == "IQU" && X4) { set(m, 0, 0, 1, 0); set(m, 0, 1, 1, 0); set(m, 1, 2, 1, 0); set(m, 2, 2, 1, 0);
+                               set(m, 3, 0, 1, 0); set(m, 3, 1, -1, 0); }
+  else if (w == "IQU" && R4) { set(m, 0, 0, 1, 0); set(m, 1, 1, 1, 0); set(m, 1, 2, 0, 1); set(m, 2, 1, 1, 0);
+                               set(m, 2, 2, 0, -1); set(m, 3, 0, 1, 0); }
+  else return fail(DDF_EINVAL, "Cannot convert input Stokes parameter to desired output correlations.");
+  sm->read = (X4 || R4) ? 4 : ((X2 || R2) ? 2 : 1);
+  sm->nstokes = v->nstokes;
+  return DDF_OK;
+}
+
+// ---- kernel parameters -----------------------------------------------------------------------
+struct VisP {
+  float2* vis;
+  const double* uvw;
+  const uint8_t* flags;
+  const float* weights;
+  const double* freqs;
+  const int32_t* chan_map;
+  int64_t nrow;
+  int nchan, ncorr;
+  const int32_t* bda;
+  const int64_t* bda_start;
+  int64_t nblocks, bda_len;
+  const uint8_t* sparse;
+};
+
+struct KParams {
+  VisP v;
+  void* grid;
+  int nband, npol, ny, nx;
+  double incr0, incr1;
+  int dopsf;
+  double cu, cv, l0, m0, n0;
+  double fmean0, wl_mean;
+  CFP cf;
+  JonesP jp;
+  DecorrP dp;
+  double* sumwt;
+  int nstokes;
+  double sm[4][4][2];
+  int chan_lds;   // per-channel Jones sums accumulated in LDS (else straight to global)
+};
+
+__device__ __forceinline__ cd smc(const KParams& p, int r, int c) { return cd{p.sm[r][c][0], p.sm[r][c][1]}; }
+
+struct BlockRec {
+  int ch0, ch1, nr;
+  const int32_t* rows;
+};
+
+__device__ __forceinline__ BlockRec read_block(const VisP& v, int64_t b) {
+  int64_t s = v.bda_start[b];
+  BlockRec r;
+  int nr = v.bda[2 + b] - 2;
+  if (s < 0 || s + 2 > v.bda_len) { r.ch0 = r.ch1 = 0; r.nr = 0; r.rows = v.bda; return r; }
+  if (nr < 0) nr = 0;
+  if (s + 2 + nr > v.bda_len) nr = int(v.bda_len - s - 2);
+  r.ch0 = clampi(v.bda[s], 0, v.nchan);
+  r.ch1 = clampi(v.bda[s + 1], r.ch0, v.nchan);
+  r.nr = nr;
+  r.rows = v.bda + s + 2;
+  return r;
+}
+
+// ---- gridding kernel -----------------------------------------------------------------------
+// Block means over the unflagged (row, channel) visibilities (gridder.h:272-289): lanes take the
+// visibilities channel-fastest and the partial sums are reduced across the wavefront.
+struct Means {
+  double su, sv, sw, sf;
+  int cnt;
+};
+
+__device__ inline Means block_means(const KParams& p, const BlockRec& br, int lane) {
+  const VisP& v = p.v;
+  const int nch = br.ch1 - br.ch0;
+  const int64_t nvis = int64_t(br.nr) * nch;
+  Means m{0, 0, 0, 0, 0};
+  for (int64_t k = lane; k < nvis; k += kWave) {
+    const int r = int(k / nch);
+    const int c = br.ch0 + int(k - int64_t(r) * nch);
+    const int64_t row = br.rows[r];
+    if (row < 0 || row >= v.nrow) continue;
+    if (v.flags[(size_t(row) * v.nchan + c) * v.ncorr]) continue;
+    const double* uvw = v.uvw + 3 * row;
+    m.su += shifted(uvw[0], uvw[2], p.cu);
+    m.sv += shifted(uvw[1], uvw[2], p.cv);
+    m.sw += uvw[2];
+    m.sf += v.freqs[c];
+    ++m.cnt;
+  }
+  m.cnt = wave_sum(m.cnt);
+  m.su = wave_sum(m.su);
+  m.sv = wave_sum(m.sv);
+  m.sw = wave_sum(m.sw);
+  m.sf = wave_sum(m.sf);
+  return m;
+}
+
+template <int READ, int JMODE, typename GT>
+__global__ __launch_bounds__(256) void grid_plain(KParams p) {
+  extern __shared__ double lds[];
+  const int nbp = p.nband * p.npol;
+  double* s_sumwt = lds;                       // [nband*npol]
+  double* s_sj = lds + nbp;                    // [2*nband]
+  double* s_sjc = lds + nbp + 2 * p.nband;     // [2*nchan] when chan_lds
+  const int nacc = nbp + 2 * p.nband + (p.chan_lds ? 2 * p.v.nchan : 0);
+  for (int i = threadIdx.x; i < nacc; i += blockDim.x) lds[i] = 0.;
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
+  const VisP& v = p.v;
+  GT* grid = reinterpret_cast<GT*>(p.grid);
+  const int npg = p.nstokes < p.npol ? p.nstokes : p.npol;
+  double* sjc = p.chan_lds ? s_sjc : p.jp.sum_jones_chan;
+
+  for (int64_t b = wave; b < v.nblocks; b += nwaves) {
+    if (v.sparse && !v.sparse[b]) continue;
+    BlockRec br = read_block(v, b);
+    const int nch = br.ch1 - br.ch0;
+    const int64_t nvis = int64_t(br.nr) * nch;
+    if (nvis <= 0) continue;
+    const int64_t mid = br.rows[br.nr / 2];
+    const bool mid_ok = mid >= 0 && mid < v.nrow;
+    const double dec = mid_ok ? decorr_factor(p.dp, v.uvw, p.fmean0, mid) : 1.;
+    const int gchan = v.chan_map[br.ch0];
+
+    // With Jones the per-channel sums are only added for gridded blocks, so the placement is
+    // decided first from a means-only pass (the same partial sums the main pass forms).
+    Place pl;
+    bool decided = false;
+    double BBblk = 0;
+    JonesPair jblk;
+    if (JMODE != 0) {
+      Means m = block_means(p, br, lane);
+      if (m.cnt == 0) continue;
+      const double um = m.su / m.cnt, vm = m.sv / m.cnt, wm = m.sw / m.cnt, fm = m.sf / m.cnt;
+      if (!(gchan >= 0 && gchan < p.nband &&
+            place_block(p.cf, p.nx, p.ny, p.incr0, p.incr1, um, vm, wm, fm / kC, &pl)))
+        continue;
+      decided = true;
+      if (JMODE == 2) {
+        double uvwm[3] = {um, vm, wm};
+        jblk = jones_at(p.jp, mid_ok ? mid : 0, (br.ch0 + br.ch1) / 2, uvwm, p.wl_mean);
+        BBblk = jblk.BB;
+      }
+    }
+
+    double su = 0, sv = 0, sw = 0, sf = 0, swt = 0, ssw = 0, sj = 0;
+    int cnt = 0;
+    cd acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int64_t k = lane; k < nvis; k += kWave) {
+      const int r = int(k / nch);
+      const int c = br.ch0 + int(k - int64_t(r) * nch);
+      const int64_t row = br.rows[r];
+      if (row < 0 || row >= v.nrow) continue;
+      const size_t vc = size_t(row) * v.nchan + c;
+      if (v.flags[vc * v.ncorr]) continue;
+      const double* uvw = v.uvw + 3 * row;
+      const double U = uvw[0], V = uvw[1], W = uvw[2];
+      const double f = v.freqs[c];
+      const double fw = double(v.weights[vc]);
+      const double fwd = fw * dec * dec;
+      cd meas[4];
+      if (p.dopsf) {
+        meas[0] = cd{dec, 0}; meas[1] = cd{0, 0}; meas[2] = cd{0, 0}; meas[3] = cd{dec, 0};
+      } else {
+        const float2* vp = v.vis + vc * v.ncorr;
+        if (READ == 4) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) meas[q] = cd{vp[q].x, vp[q].y};
+        } else if (READ == 2) {
+          meas[0] = cd{vp[0].x, vp[0].y}; meas[1] = cd{0, 0}; meas[2] = cd{0, 0}; meas[3] = cd{vp[1].x, vp[1].y};
+        } else {
+          meas[0] = cd{vp[0].x, vp[0].y}; meas[1] = cd{0, 0}; meas[2] = cd{0, 0}; meas[3] = meas[0];
+        }
+      }
+      cd wt = cd{fw, 0};
+      if (!p.dopsf) {
+        const double angle = -2. * kPi * (U * p.l0 + V * p.m0 + W * p.n0) / kC;
+        wt = phasor(f * angle) * fw;
+      }
+      if (JMODE == 1) {
+        JonesPair jj = jones_at(p.jp, row, c, uvw, p.wl_mean);
+        M2 m;
+        if (p.dopsf) {
+          m = mul(mul(herm(jj.J0), mul(jj.J0, herm(jj.J1))), jj.J1);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) m.a[q] = m.a[q] * dec;
+        } else {
+          m = mul(mul(herm(jj.J0), M2{{meas[0], meas[1], meas[2], meas[3]}}), jj.J1);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) meas[q] = m.a[q];
+        sj += jj.BB * fwd;
+        atomicAdd(&sjc[c], double(npg) * (jj.BB * fwd));
+      }
+      if (JMODE == 2) atomicAdd(&sjc[c], double(npg) * (fwd * BBblk));
+      if (JMODE != 0) atomicAdd(&sjc[v.nchan + c], double(npg) * fwd);
+      if (READ == 4) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = acc[q] + meas[q] * wt;
+      } else if (READ == 2) {
+        acc[0] = acc[0] + meas[0] * wt;
+        acc[3] = acc[3] + meas[3] * wt;
+      } else {
+        acc[0] = acc[0] + meas[0] * wt;
+      }
+      if (JMODE == 0) {
+        su += shifted(U, W, p.cu);
+        sv += shifted(V, W, p.cv);
+        sw += W;
+        sf += f;
+        ++cnt;
+      }
+      swt += fw;
+      ssw += fwd;
+    }
+    if (JMODE == 0) {
+      cnt = wave_sum(cnt);
+      if (cnt == 0) continue;
+      su = wave_sum(su); sv = wave_sum(sv); sw = wave_sum(sw); sf = wave_sum(sf);
+      decided = gchan >= 0 && gchan < p.nband &&
+                place_block(p.cf, p.nx, p.ny, p.incr0, p.incr1, su / cnt, sv / cnt, sw / cnt, (sf / cnt) / kC, &pl);
+      if (!decided) continue;
+    }
+    swt = wave_sum(swt);
+    ssw = wave_sum(ssw);
+    if (JMODE == 1) sj = wave_sum(sj);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (READ == 4 || q == 0 || (READ == 2 && q == 3)) {
+        acc[q].re = wave_sum(acc[q].re);
+        acc[q].im = wave_sum(acc[q].im);
+      }
+    }
+    if (JMODE == 2) {
+      M2 m{{acc[0], acc[1], acc[2], acc[3]}};
+      if (p.dopsf) m = mul(mul(jblk.J0, m), herm(jblk.J1));
+      m = mul(mul(herm(jblk.J0), m), jblk.J1);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = m.a[q];
+      sj = ssw * BBblk;
+    }
+    // Stokes conversion
+    cd st[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      st[k] = cd{0, 0};
+      if (k < p.nstokes)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) st[k] = st[k] + smc(p, k, q) * acc[q];
+    }
+    const int S = pl.S, S2 = S * S;
+    const float2* cfp = p.cf.data + pl.cfoff;
+    for (int t = lane; t < S2 * npg; t += kWave) {
+      const int ip = t / S2;
+      const int k = t - ip * S2;
+      const int sy = k / S, sx = k - sy * S;
+      const float2 w = cfp[k];
+      cd val = ip == 0 ? st[0] : (ip == 1 ? st[1] : (ip == 2 ? st[2] : st[3]));
+      cd prod = val * cd{double(w.x), double(w.y)};
+      size_t gi = ((size_t(gchan) * p.npol + ip) * p.ny + size_t(pl.locy - pl.sup + sy)) * p.nx +
+                  size_t(pl.locx - pl.sup + sx);
+      atomicAdd(&grid[2 * gi], GT(prod.re));
+      atomicAdd(&grid[2 * gi + 1], GT(prod.im));
+    }
+    if (lane == 0) {
+      for (int ip = 0; ip < npg; ++ip) {
+        atomicAdd(&s_sumwt[gchan * p.npol + ip], swt);
+        if (JMODE != 0) {
+          atomicAdd(&s_sj[gchan], sj);
+          atomicAdd(&s_sj[gchan + p.nband], ssw);
+        }
+      }
+    }
+    (void)decided;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nbp; i += blockDim.x)
+    if (s_sumwt[i] != 0.) atomicAdd(&p.sumwt[i], s_sumwt[i]);
+  if (JMODE != 0) {
+    for (int i = threadIdx.x; i < 2 * p.nband; i += blockDim.x)
+      if (s_sj[i] != 0.) atomicAdd(&p.jp.sum_jones[i], s_sj[i]);
+    if (p.chan_lds)
+      for (int i = threadIdx.x; i < 2 * p.v.nchan; i += blockDim.x)
+        if (s_sjc[i] != 0.) atomicAdd(&p.jp.sum_jones_chan[i], s_sjc[i]);
+  }
+}
+
+// ---- degridding kernel ---------------------------------------------------------------------
+template <int NCORR, int JMODE>
+__global__ __launch_bounds__(256) void degrid_plain(KParams p) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
+  const VisP& v = p.v;
+  const float2* grid = reinterpret_cast<const float2*>(p.grid);
+
+  for (int64_t b = wave; b < v.nblocks; b += nwaves) {
+    BlockRec br = read_block(v, b);
+    const int nch = br.ch1 - br.ch0;
+    if (br.nr <= 0 || nch <= 0) continue;
+    const int gchan = v.chan_map[br.ch0];
+    if (gchan < 0 || gchan >= p.nband) continue;
+    double fm = 0;
+    for (int c = br.ch0; c < br.ch1; ++c) fm += v.freqs[c];
+    fm /= double(nch);
+    double su = 0, sv = 0, sw = 0;
+    int cnt = 0;
+    for (int r = lane; r < br.nr; r += kWave) {
+      const int64_t row = br.rows[r];
+      if (row < 0 || row >= v.nrow) continue;
+      const double* uvw = v.uvw + 3 * row;
+      su += shifted(uvw[0], uvw[2], p.cu);
+      sv += shifted(uvw[1], uvw[2], p.cv);
+      sw += uvw[2];
+      ++cnt;
+    }
+    cnt = wave_sum(cnt);
+    if (cnt == 0) continue;
+    su = wave_sum(su); sv = wave_sum(sv); sw = wave_sum(sw);
+    Place pl;
+    if (!place_block(p.cf, p.nx, p.ny, p.incr0, p.incr1, su / cnt, sv / cnt, sw / cnt, fm / kC, &pl)) continue;
+    // gather the S x S stamp of every Stokes plane: FP32 products summed in FP64 (degridder.h:202-212)
+    const int S = pl.S, S2 = S * S;
+    const float2* cfp = p.cf.data + pl.cfoff;
+    cd st[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int t = lane; t < S2 * p.nstokes; t += kWave) {
+      const int ip = t / S2;
+      const int k = t - ip * S2;
+      const int sy = k / S, sx = k - sy * S;
+      const size_t gi = ((size_t(gchan) * p.npol + ip) * p.ny + size_t(pl.locy - pl.sup + sy)) * p.nx +
+                        size_t(pl.locx - pl.sup + sx);
+      const float2 g = grid[gi];
+      const float2 w = cfp[k];
+      const float pr = g.x * w.x - g.y * w.y;
+      const float pi = g.x * w.y + g.y * w.x;
+      if (ip == 0) st[0] = st[0] + cd{pr, pi};
+      else if (ip == 1) st[1] = st[1] + cd{pr, pi};
+      else if (ip == 2) st[2] = st[2] + cd{pr, pi};
+      else st[3] = st[3] + cd{pr, pi};
+    }
+    for (int k = 0; k < p.nstokes; ++k) {
+      st[k].re = wave_sum(st[k].re);
+      st[k].im = wave_sum(st[k].im);
+    }
+    M2 cv;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      cv.a[k] = cd{0, 0};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) cv.a[k] = cv.a[k] + smc(p, k, q) * st[q];
+    }
+    const int64_t mid = br.rows[br.nr / 2];
+    const bool mid_ok = mid >= 0 && mid < v.nrow;
+    if (JMODE == 2 && mid_ok) {
+      JonesPair jj = jones_at(p.jp, mid, (br.ch0 + br.ch1) / 2, v.uvw + 3 * mid, p.wl_mean);
+      cv = mul(mul(jj.J0, cv), herm(jj.J1));
+    }
+    const double dec = mid_ok ? decorr_factor(p.dp, v.uvw, fm, mid) : 1.;
+    const int64_t nvis = int64_t(br.nr) * nch;
+    for (int64_t k = lane; k < nvis; k += kWave) {
+      const int r = int(k / nch);
+      const int c = br.ch0 + int(k - int64_t(r) * nch);
+      const int64_t row = br.rows[r];
+      if (row < 0 || row >= v.nrow) continue;
+      const double* uvw = v.uvw + 3 * row;
+      const double angle = 2. * kPi * (uvw[0] * p.l0 + uvw[1] * p.m0 + uvw[2] * p.n0) / kC;
+      const cd corr = phasor(v.freqs[c] * angle) * dec;
+      M2 vb;
+      if (JMODE == 1) {
+        JonesPair jj = jones_at(p.jp, row, c, uvw, p.wl_mean);
+        vb = mul(mul(jj.J0, cv), herm(jj.J1));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) vb.a[q] = vb.a[q] * corr;
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) vb.a[q] = cv.a[q] * corr;
+      }
+      float2* vp = v.vis + (size_t(row) * v.nchan + c) * NCORR;
+      if (NCORR == 4) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          float2 x = vp[q];
+          vp[q] = make_float2(float(double(x.x) - vb.a[q].re), float(double(x.y) - vb.a[q].im));
+        }
+      } else if (NCORR == 2) {
+        float2 x0 = vp[0], x1 = vp[1];
+        vp[0] = make_float2(float(double(x0.x) - vb.a[0].re), float(double(x0.y) - vb.a[0].im));
+        vp[1] = make_float2(float(double(x1.x) - vb.a[3].re), float(double(x1.y) - vb.a[3].im));
+      } else {
+        float2 x0 = vp[0];
+        vp[0] = make_float2(float(double(x0.x) - vb.a[0].re), float(double(x0.y) - vb.a[0].im));
+      }
+    }
+  }
+}
+
+// ---- BDA block index ---------------------------------------------------------------------------
+// Word offset of every block record: start[b] = 2 + nblocks + sum_{b' < b} size[b'].
+// Three phases: per-chunk sums, an exclusive scan of the chunk sums (one workgroup), and a
+// per-chunk scan plus chunk offset.
+constexpr int kScanChunk = 2048;
+
+__global__ __launch_bounds__(256) void scan_chunk_sums(const int32_t* sizes, int64_t n, int64_t* chunk_sum) {
+  __shared__ int64_t red[256];
+  int64_t base = int64_t(blockIdx.x) * kScanChunk;
+  int64_t s = 0;
+  for (int i = threadIdx.x; i < kScanChunk; i += 256)
+    if (base + i < n) s += sizes[base + i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) chunk_sum[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void scan_chunk_offsets(int64_t* chunk_sum, int64_t nchunk, int64_t base) {
+  // exclusive scan of chunk_sum in place, one workgroup, sequential over tiles of 256
+  __shared__ int64_t tile[256];
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = base;
+  __syncthreads();
+  for (int64_t t0 = 0; t0 < nchunk; t0 += 256) {
+    int64_t i = t0 + threadIdx.x;
+    int64_t x = i < nchunk ? chunk_sum[i] : 0;
+    tile[threadIdx.x] = x;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+      int64_t y = threadIdx.x >= o ? tile[threadIdx.x - o] : 0;
+      __syncthreads();
+      tile[threadIdx.x] += y;
+      __syncthreads();
+    }
+    if (i < nchunk) chunk_sum[i] = carry + tile[threadIdx.x] - x;
+    __syncthreads();
+    if (threadIdx.x == 255) carry += tile[255];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void scan_chunk_final(const int32_t* sizes, int64_t n, const int64_t* chunk_off,
+                                                        int64_t* out) {
+  __shared__ int64_t tile[256];
+  __shared__ int64_t carry;
+  int64_t base = int64_t(blockIdx.x) * kScanChunk;
+  if (threadIdx.x == 0) carry = chunk_off[blockIdx.x];
+  __syncthreads();
+  for (int t0 = 0; t0 < kScanChunk; t0 += 256) {
+    int64_t i = base + t0 + threadIdx.x;
+    int64_t x = i < n ? sizes[i] : 0;
+    tile[threadIdx.x] = x;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+      int64_t y = threadIdx.x >= o ? tile[threadIdx.x - o] : 0;
+      __syncthreads();
+      tile[threadIdx.x] += y;
+      __syncthreads();
+    }
+    if (i < n) out[i] = carry + tile[threadIdx.x] - x;
+    __syncthreads();
+    if (threadIdx.x == 255) carry += tile[255];
+    __syncthreads();
+  }
+}
+
+int launch_bda_index(const int32_t* bda, int64_t nblocks, int64_t* out, int64_t* chunk_tmp, hipStream_t st) {
+  if (nblocks <= 0) return DDF_OK;
+  int64_t nchunk = (nblocks + kScanChunk - 1) / kScanChunk;
+  hipLaunchKernelGGL(scan_chunk_sums, dim3(unsigned(nchunk)), dim3(256), 0, st, bda + 2, nblocks, chunk_tmp);
+  hipLaunchKernelGGL(scan_chunk_offsets, dim3(1), dim3(256), 0, st, chunk_tmp, nchunk, 2 + nblocks);
+  hipLaunchKernelGGL(scan_chunk_final, dim3(unsigned(nchunk)), dim3(256), 0, st, bda + 2, nblocks, chunk_tmp, out);
+  HIP_TRY(hipGetLastError());
+  return DDF_OK;
+}
+
+// ---- host-side assembly ---------------------------------------------------------------------
+int check_common(const ddf_grid_desc* g, const ddf_vis* v, const ddf_facet* f) {
+  if (!g || !v || !f) return fail(DDF_EINVAL, "null argument struct");
+  if (g->nband <= 0 || g->npol <= 0 || g->nx <= 0 || g->ny <= 0) return fail(DDF_EINVAL, "bad grid shape");
+  if (!f->grid) return fail(DDF_EINVAL, "grid pointer is null");
+  if (v->nrow < 0 || v->nchan <= 0 || v->ncorr <= 0 || v->ncorr > 4) return fail(DDF_EINVAL, "bad visibility shape");
+  if (v->ncorr != v->ncorr_codes) return fail(DDF_EINVAL, "DataCorrFormat must list one code per correlation");
+  if (v->nstokes <= 0 || v->nstokes > 4) return fail(DDF_EINVAL, "bad number of Stokes parameters");
+  if (!v->uvw || !v->vis || !v->flags || !v->freqs || !v->chan_map || !v->bda)
+    return fail(DDF_EINVAL, "null visibility-side pointer");
+  if (v->bda_len < 2 || v->nblocks < 0 || 2 + v->nblocks > v->bda_len) return fail(DDF_EINVAL, "inconsistent BDA map");
+  const ddf_cf& cf = f->cf;
+  if (!cf.data || !cf.offset || !cf.support || cf.nw <= 0 || cf.overs <= 0 || cf.max_support <= 0)
+    return fail(DDF_EINVAL, "bad CF table");
+  if (f->jones.enabled) {
+    const ddf_jones& j = f->jones;
+    if (!j.kms && !j.beam)
+      return fail(DDF_ERUNTIME, "Jones matricies specified but neither E or DD Jones are applied. This is a bug!");
+    if (j.mode != 1 && j.mode != 2) return fail(DDF_EINVAL, "Jones mode must be 1 or 2");
+    if (!j.a0 || !j.a1) return fail(DDF_EINVAL, "Jones antenna maps are null");
+    if (j.kms && (!j.kms_time || !j.kms_chan)) return fail(DDF_EINVAL, "DD Jones maps are null");
+    if (j.beam && (!j.beam_time || !j.beam_chan)) return fail(DDF_EINVAL, "beam Jones maps are null");
+    for (int i = 0; i < 4; ++i)
+      if ((j.kms && j.kms_dims[i] <= 0) || (j.beam && j.beam_dims[i] <= 0)) return fail(DDF_EINVAL, "empty Jones table");
+    if (j.interp_mode == 1 && j.ncoefs > 0 && !j.coefs) return fail(DDF_EINVAL, "interpolation coefficients are null");
+  }
+  if (f->decorr.enabled && f->decorr.tsmear && !f->decorr.uvw_dt) return fail(DDF_EINVAL, "uvw_dt is null");
+  return DDF_OK;
+}
+
+void fill_params(KParams& p, const ddf_grid_desc* g, const ddf_vis* v, const ddf_facet* f, const StokesMap& sm,
+                 const int64_t* bda_start, const std::vector<double>& freqs_host) {
+  std::memset(&p, 0, sizeof(p));
+  p.v.vis = reinterpret_cast<float2*>(v->vis);
+  p.v.uvw = v->uvw;
+  p.v.flags = v->flags;
+  p.v.weights = v->weights;
+  p.v.freqs = v->freqs;
+  p.v.chan_map = v->chan_map;
+  p.v.nrow = v->nrow;
+  p.v.nchan = v->nchan;
+  p.v.ncorr = v->ncorr;
+  p.v.bda = v->bda;
+  p.v.bda_start = bda_start;
+  p.v.nblocks = v->nblocks;
+  p.v.bda_len = v->bda_len;
+  p.v.sparse = v->sparsification;
+  p.grid = f->grid;
+  p.nband = g->nband;
+  p.npol = g->npol;
+  p.ny = g->ny;
+  p.nx = g->nx;
+  p.incr0 = g->incr[0];
+  p.incr1 = g->incr[1];
+  p.dopsf = g->dopsf;
+  p.cu = f->cu;
+  p.cv = f->cv;
+  p.l0 = f->l0;
+  p.m0 = f->m0;
+  p.n0 = std::sqrt(1 - f->l0 * f->l0 - f->m0 * f->m0) - 1;
+  double wl = 0, fm = 0;
+  for (double x : freqs_host) {
+    wl += kC / x;
+    fm += x;
+  }
+  p.wl_mean = wl / double(freqs_host.size());
+  p.fmean0 = fm / double(freqs_host.size());
+  p.cf.data = reinterpret_cast<const float2*>(f->cf.data);
+  p.cf.offset = f->cf.offset;
+  p.cf.support = f->cf.support;
+  p.cf.nw = f->cf.nw;
+  p.cf.overs = f->cf.overs;
+  p.cf.ref_wave = f->cf.ref_wave;
+  p.cf.wmax = f->cf.wmax;
+  const ddf_jones& j = f->jones;
+  p.jp.enabled = j.enabled;
+  p.jp.mode = j.mode;
+  p.jp.interp = j.interp_mode;
+  p.jp.kms = reinterpret_cast<const float2*>(j.kms);
+  p.jp.beam = reinterpret_cast<const float2*>(j.beam);
+  for (int i = 0; i < 4; ++i) {
+    p.jp.kdims[i] = j.kms_dims[i];
+    p.jp.bdims[i] = j.beam_dims[i];
+  }
+  p.jp.ktime = j.kms_time;
+  p.jp.kchan = j.kms_chan;
+  p.jp.kdir = j.kms_dir;
+  p.jp.btime = j.beam_time;
+  p.jp.bchan = j.beam_chan;
+  p.jp.bdir = j.beam_dir;
+  p.jp.a0 = j.a0;
+  p.jp.a1 = j.a1;
+  p.jp.coefs = j.coefs;
+  p.jp.ncoefs = j.coefs ? j.ncoefs : 0;
+  p.jp.apply_amp = j.apply_amp;
+  p.jp.apply_phase = j.apply_phase;
+  p.jp.scale = j.scale_jones;
+  p.jp.calib_error = j.calib_error;
+  p.jp.sum_jones = j.sum_jones;
+  p.jp.sum_jones_chan = j.sum_jones_chan;
+  p.dp.enabled = f->decorr.enabled;
+  p.dp.tsmear = f->decorr.tsmear;
+  p.dp.fsmear = f->decorr.fsmear;
+  p.dp.uvw_dt = f->decorr.uvw_dt;
+  p.dp.dt = f->decorr.dt;
+  p.dp.dnu = f->decorr.dnu;
+  p.dp.l0 = f->decorr.l0;
+  p.dp.m0 = f->decorr.m0;
+  p.sumwt = f->sumwt;
+  p.nstokes = sm.nstokes;
+  std::memcpy(p.sm, sm.m, sizeof(p.sm));
+}
+
+// Per-call temporary device memory (stream ordered).
+struct Temp {
+  hipStream_t st;
+  std::vector<void*> ptrs;
+  explicit Temp(hipStream_t s) : st(s) {}
+  void* get(size_t bytes) {
+    void* p = nullptr;
+    if (hipMallocAsync(&p, bytes, st) != hipSuccess) return nullptr;
+    ptrs.push_back(p);
+    return p;
+  }
+  ~Temp() {
+    for (void* p : ptrs) (void)hipFreeAsync(p, st);
+  }
+};
+
+int resolve_block_index(const ddf_vis* v, Temp& tmp, const int64_t** out) {
+  if (v->bda_start) {
+    *out = v->bda_start;
+    return DDF_OK;
+  }
+  if (v->nblocks == 0) {
+    *out = nullptr;
+    return DDF_OK;
+  }
+  int64_t nchunk = (v->nblocks + kScanChunk - 1) / kScanChunk;
+  int64_t* starts = static_cast<int64_t*>(tmp.get(size_t(v->nblocks) * 8));
+  int64_t* chunks = static_cast<int64_t*>(tmp.get(size_t(nchunk) * 8));
+  if (!starts || !chunks) return fail(DDF_ENOMEM, "device allocation of the BDA block index failed");
+  int rc = launch_bda_index(v->bda, v->nblocks, starts, chunks, tmp.st);
+  *out = starts;
+  return rc;
+}
+
+int host_freqs(const ddf_vis* v, std::vector<double>& f, hipStream_t st) {
+  f.resize(size_t(v->nchan));
+  HIP_TRY(hipMemcpyAsync(f.data(), v->freqs, f.size() * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return DDF_OK;
+}
+
+unsigned grid_blocks(int64_t nblocks) {
+  int64_t need = (nblocks + 3) / 4;
+  int64_t cap = 256 * 16;
+  if (need < 1) need = 1;
+  return unsigned(need < cap ? need : cap);
+}
+
+template <int READ, int JMODE>
+int launch_grid_t(const KParams& p, int grid_f64, size_t lds, unsigned nb, hipStream_t st) {
+  if (grid_f64)
+    hipLaunchKernelGGL((grid_plain<READ, JMODE, double>), dim3(nb), dim3(256), lds, st, p);
+  else
+    hipLaunchKernelGGL((grid_plain<READ, JMODE, float>), dim3(nb), dim3(256), lds, st, p);
+  return DDF_OK;
+}
+
+template <int READ>
+int launch_grid_r(const KParams& p, int jmode, int grid_f64, size_t lds, unsigned nb, hipStream_t st) {
+  if (jmode == 1) return launch_grid_t<READ, 1>(p, grid_f64, lds, nb, st);
+  if (jmode == 2) return launch_grid_t<READ, 2>(p, grid_f64, lds, nb, st);
+  return launch_grid_t<READ, 0>(p, grid_f64, lds, nb, st);
+}
+
+template <int NCORR>
+int launch_degrid_r(const KParams& p, int jmode, unsigned nb, hipStream_t st) {
+  if (jmode == 1) hipLaunchKernelGGL((degrid_plain<NCORR, 1>), dim3(nb), dim3(256), 0, st, p);
+  else if (jmode == 2) hipLaunchKernelGGL((degrid_plain<NCORR, 2>), dim3(nb), dim3(256), 0, st, p);
+  else hipLaunchKernelGGL((degrid_plain<NCORR, 0>), dim3(nb), dim3(256), 0, st, p);
+  return DDF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ddf_version(void) { return DDF_ABI_VERSION; }
+
+const char* ddf_last_error(void) { return g_err.c_str(); }
+
+int ddf_init(int device) {
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(hipFree(nullptr));
+  return DDF_OK;
+}
+
+int ddf_bda_index(const int32_t* bda, int64_t bda_len, int64_t nblocks, int64_t* out, void* stream) {
+  if (!bda || !out || nblocks < 0 || 2 + nblocks > bda_len) return fail(DDF_EINVAL, "inconsistent BDA map");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Temp tmp(st);
+  int64_t nchunk = (nblocks + kScanChunk - 1) / kScanChunk;
+  int64_t* chunks = nchunk ? static_cast<int64_t*>(tmp.get(size_t(nchunk) * 8)) : nullptr;
+  if (nchunk && !chunks) return fail(DDF_ENOMEM, "device allocation failed");
+  return launch_bda_index(bda, nblocks, out, chunks, st);
+}
+
+int ddf_grid(const ddf_grid_desc* g, const ddf_vis* v, const ddf_facet* f, void* stream) {
+  int rc = check_common(g, v, f);
+  if (rc) return rc;
+  if (!v->weights || !f->sumwt) return fail(DDF_EINVAL, "weights / sumwt pointer is null");
+  if (f->jones.enabled && (!f->jones.sum_jones || !f->jones.sum_jones_chan))
+    return fail(DDF_EINVAL, "SumJones / SumJonesChan pointer is null");
+  StokesMap sm;
+  rc = grid_map(v, &sm);
+  if (rc) return rc;
+  if (v->nblocks == 0 || v->nrow == 0) return DDF_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  std::vector<double> fh;
+  rc = host_freqs(v, fh, st);
+  if (rc) return rc;
+  Temp tmp(st);
+  const int64_t* starts = nullptr;
+  rc = resolve_block_index(v, tmp, &starts);
+  if (rc) return rc;
+  KParams p;
+  fill_params(p, g, v, f, sm, starts, fh);
+  const int jmode = f->jones.enabled ? f->jones.mode : 0;
+  size_t lds_base = size_t(g->nband) * g->npol + 2 * size_t(g->nband);
+  p.chan_lds = (jmode != 0) && (lds_base + 2 * size_t(v->nchan)) * 8 <= 48 * 1024;
+  size_t lds = (lds_base + (p.chan_lds ? 2 * size_t(v->nchan) : 0)) * 8;
+  if (lds > 60 * 1024) return fail(DDF_EINVAL, "nband * npol too large for the gridding accumulators");
+  unsigned nb = grid_blocks(v->nblocks);
+  if (sm.read == 4) rc = launch_grid_r<4>(p, jmode, g->grid_f64, lds, nb, st);
+  else if (sm.read == 2) rc = launch_grid_r<2>(p, jmode, g->grid_f64, lds, nb, st);
+  else rc = launch_grid_r<1>(p, jmode, g->grid_f64, lds, nb, st);
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError());
+  return DDF_OK;
+}
+
+int ddf_degrid(const ddf_grid_desc* g, const ddf_vis* v, const ddf_facet* f, void* stream) {
+  int rc = check_common(g, v, f);
+  if (rc) return rc;
+  if (g->grid_f64) return fail(DDF_EINVAL, "the degridder reads complex64 grids");
+  StokesMap sm;
+  rc = degrid_map(v, &sm);
+  if (rc) return rc;
+  if (sm.nstokes > g->npol) return fail(DDF_EINVAL, "grid has fewer polarisations than the requested Stokes parameters");
+  if (v->nblocks == 0 || v->nrow == 0) return DDF_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  std::vector<double> fh;
+  rc = host_freqs(v, fh, st);
+  if (rc) return rc;
+  Temp tmp(st);
+  const int64_t* starts = nullptr;
+  rc = resolve_block_index(v, tmp, &starts);
+  if (rc) return rc;
+  KParams p;
+  fill_params(p, g, v, f, sm, starts, fh);
+  const int jmode = f->jones.enabled ? f->jones.mode : 0;
+  unsigned nb = grid_blocks(v->nblocks);
+  if (v->ncorr == 4) rc = launch_degrid_r<4>(p, jmode, nb, st);
+  else if (v->ncorr == 2) rc = launch_degrid_r<2>(p, jmode, nb, st);
+  else if (v->ncorr == 1) rc = launch_degrid_r<1>(p, jmode, nb, st);
+  else return fail(DDF_EINVAL, "ncorr must be 1, 2 or 4");
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError());
+  return DDF_OK;
+}
+
+}  // extern "C"
+
+// ---- weights scatter-add ---------------------------------------------------------------------
+namespace {
+__global__ __launch_bounds__(256) void accumulate_weights(double* grid, int64_t glen, const float* w, const int64_t* idx,
+                                                          int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    float x = w[i];
+    if (x != 0) {
+      int64_t k = idx[i];
+      if (k >= 0 && k < glen) atomicAdd(&grid[k], double(x));
+    }
+  }
+}
+}  // namespace
+
+extern "C" int ddf_accumulate_weights(double* grid, int64_t grid_len, const float* weights, const int64_t* index,
+                                      int64_t n, void* stream) {
+  if (n < 0 || grid_len < 0) return fail(DDF_EINVAL, "negative size");
+  if (n == 0) return DDF_OK;
+  if (!grid || !weights || !index) return fail(DDF_EINVAL, "null pointer");
+  int64_t nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(accumulate_weights, dim3(unsigned(nb)), dim3(256), 0, static_cast<hipStream_t>(stream), grid,
+                     grid_len, weights, index, n);
+  HIP_TRY(hipGetLastError());
+  return DDF_OK;
+}

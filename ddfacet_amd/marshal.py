@@ -138,6 +138,23 @@ def parse_jones(ljones, grid_side):
         sum_jones=sum_jones, sum_jones_chan=sum_jones_chan, reweight_snr=float(ljones[19]))
 
 
+def _check_jones(j, nrow, nchan, nband):
+    """Lengths the device code indexes with (the reference reads them unchecked)."""
+    if j.a0.size != nrow or j.a1.size != nrow:
+        raise RuntimeError("Antenna array is expected to have the same number of rows as the uvw array")
+    for tab, tmap, cmap, what in ((j.kms, j.kms_time, j.kms_chan, "DD"), (j.beam, j.beam_time, j.beam_chan, "beam")):
+        if tab is None:
+            continue
+        if tmap.size != nrow:
+            raise ValueError("%s Jones time mapping must have one entry per row" % what)
+        if cmap.size != nchan:
+            raise ValueError("%s Jones channel mapping must have one entry per channel" % what)
+    if j.mode not in (1, 2):
+        raise ValueError("BDAJones mode must be 1 or 2")
+    if j.sum_jones.size < 2 * nband or j.sum_jones_chan.size < 2 * nchan:
+        raise ValueError("SumJones must be [2, nband] and SumJonesChan [2, nchan]")
+
+
 @dataclass
 class CallSpec:
     grid: np.ndarray
@@ -206,10 +223,15 @@ def _parse_common(grid, vis, uvw, flags, dopsf, lcfs, lcfs_conj, winfos, increme
         raise ValueError("Cannot convert input correlations to desired output Stokes parameters.")
     if corr.size != vis.shape[2]:
         raise ValueError("DataCorrFormat must list one code per correlation")
+    jones = parse_jones(ljones, grid_side)
+    if jones is not None:
+        _check_jones(jones, vis.shape[0], vis.shape[1], grid.shape[0])
+    if decorr is not None and decorr[0].shape != uvw.shape:
+        raise ValueError("LSmearing[0] (uvw_dt) must be [nrow, 3]")
     return dict(
         uvw=uvw, flags=flags, dopsf=int(bool(dopsf)), cf=pack_cfs(lcfs, lcfs_conj, winfos),
         incr=_as(increment, np.float64), freqs=freqs, facet_infos=_as(lmaps[1], np.float64),
-        jones=parse_jones(ljones, grid_side), bda=bda, nblocks=nblocks, sparsification=sp,
+        jones=jones, bda=bda, nblocks=nblocks, sparsification=sp,
         chan_equidistant=int(bool(loptimisation[1])), decorr=decorr, chan_map=chan_map,
         corr_codes=corr, stokes_codes=stokes)
 
