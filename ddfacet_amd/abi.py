@@ -94,6 +94,8 @@ class ddf_vis(C.Structure):
         ("ncorr_codes", C.c_int32),
         ("stokes_codes", C.c_uint16 * 4),
         ("nstokes", C.c_int32),
+        ("freq_mean", C.c_double),
+        ("wavelength_mean", C.c_double),
     ]
 
 
@@ -125,3 +127,47 @@ class ddf_grid_desc(C.Structure):
 
 STOKES_CODES = {"undef": 0, "I": 1, "Q": 2, "U": 3, "V": 4, "RR": 5, "RL": 6, "LR": 7, "LL": 8,
                 "XX": 9, "XY": 10, "YX": 11, "YY": 12}
+
+
+class ddf_plan_opts(C.Structure):
+    _fields_ = [
+        ("tile", C.c_int32),
+        ("chunk", C.c_int32),
+        ("blocks_per_item", C.c_int32),
+        ("group", C.c_int32),
+        ("precision", C.c_int32),
+        ("workgroups", C.c_int32),
+    ]
+
+
+class ddf_image_facet(C.Structure):
+    _fields_ = [
+        ("xc", C.c_int32),
+        ("yc", C.c_int32),
+        ("sphe", P),
+        ("inv_sphe", P),
+        ("sw", P),
+        ("dirty", P),
+        ("model", P),
+        ("sumwt", P),
+        ("sum_jones", P),
+    ]
+
+
+PRECISION = {"mixed": 0, "fp64": 1, "fp32": 2}
+
+EXTRA_SIGNATURES = {
+    "ddf_plan_create": [C.POINTER(ddf_grid_desc), C.POINTER(ddf_vis), C.POINTER(ddf_vis), C.POINTER(ddf_facet),
+                        C.POINTER(C.c_void_p), C.c_int32, C.POINTER(ddf_plan_opts), C.POINTER(C.c_void_p),
+                        C.c_void_p],
+    "ddf_plan_grid": [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p],
+    "ddf_plan_degrid": [C.c_void_p, C.c_void_p, C.c_void_p],
+    "ddf_plan_info": [C.c_void_p, C.POINTER(C.c_int64), C.c_int32],
+    "ddf_plan_destroy": [C.c_void_p],
+    "ddf_fft2c": [C.c_void_p, C.c_int32, C.c_int64, C.c_int32, C.c_double, C.c_void_p],
+    "ddf_facet_norm": [C.POINTER(ddf_image_facet), C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p],
+    "ddf_stitch": [C.POINTER(ddf_image_facet), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                   C.c_void_p, C.c_void_p],
+    "ddf_model_to_grid": [C.POINTER(ddf_image_facet), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                          C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p],
+}

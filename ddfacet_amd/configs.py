@@ -1,0 +1,143 @@
+"""The benchmark configurations as concrete problems.
+
+cfg1  plain single facet: 27 antennas in a 20 km disk, 351 baselines, 100 x 10 s, 16 channels
+      1.2-1.6 GHz, 512 x 512 grid at 2", Nw = 8, S = 7, OverS = 11, identity Jones, one BDA
+      block per (row, channel), 10 point sources + noise, 1% flags.
+cfg2  81-facet MeerKAT-like: 64 antennas (dense 1 km core + 3 arms to 8 km), 2016 baselines,
+      600 x 8 s, 64 channels 0.9-1.67 GHz, 4096 x 4096 image at 1.5" in 9 x 9 facets of 455 px
+      padded to 735 (EstimateNpix with Padding 1.7), Nw = 32 with supports odd(linspace(7, 15)),
+      OverS = 11, per-visibility DD Jones [8 x 81 x 64 x 8] (10-min x 8-channel bins, nearest
+      facet direction), BDA maps of the default builder (GridDecorr = DegridDecorr = 0.02, facet
+      FoV), 200 power-law point sources + noise, 1% flags.
+Seeds: 1000 * cfg + k (k = 0 array, 1 sky, 2 noise, 3 flags, 4 Jones).
+"""
+import time
+
+import numpy as np
+
+from . import bda as bdamod
+from . import synth
+from .problems import Facet, Problem, chan_equidistant, make_cf_facet
+
+C_LIGHT = 299792458.0
+
+# odd fast FFT sizes (factors 3, 5, 7, optionally one 11 or 13), ModToolBox.py:273-293
+def _fast_odd_sizes(nlim=200000):
+    sizes = np.array([1])
+    for base in (3, 5, 7):
+        sizes = (sizes[None, :] * base ** np.arange(15)[:, None]).ravel()
+    sizes = (sizes[None, :] * np.array([1, 11, 13])[:, None]).ravel()
+    return np.array(sorted(sizes[(sizes < nlim) & (sizes > 64)]))
+
+
+_ODD = _fast_odd_sizes()
+
+
+def estimate_npix(npix, padding=1.0, min_size_fft=513):
+    """EstimateNpix (DDFacet/ToolsDir/ModToolBox.py:64-88): closest odd fast size of the facet,
+    then of the padded facet (at least min_size_fft)."""
+    n = int(round(npix))
+    n = int(_ODD[np.argmin(np.abs(n - _ODD))])
+    nopt = n
+    n = n * padding
+    if n < min_size_fft:
+        n = min_size_fft
+    n = int(round(n))
+    n = int(_ODD[np.argmin(np.abs(n - _ODD))])
+    return nopt, n
+
+
+def cfg1(noise=0.01, log=print):
+    t0 = time.time()
+    nchan = 16
+    freqs = np.linspace(1.2e9, 1.6e9, nchan)
+    obs = synth.make_observation("disk", 27, 100, 10.0, freqs, 45.0, -30.7, 1000, diameter_m=20000.0)
+    rng = np.random.default_rng(1001)
+    cell_as = 2.0
+    cell = cell_as * synth.ARCSEC
+    npix = 512
+    sky = synth.sky_loguniform(10, 0.45 * npix * cell, rng)
+    R = obs.nrow
+    vis = synth.predict_point_sources(obs.uvw, freqs, sky).numpy()
+    nr = np.random.default_rng(1002)
+    vis += (noise / np.sqrt(2)) * (nr.standard_normal(vis.shape) + 1j * nr.standard_normal(vis.shape)).astype(np.complex64)
+    flags = synth.random_flags(R, nchan, 4, 0.01, 1003)
+    weights = np.ones((R, nchan), np.float32)
+    wmax = float(np.abs(obs.uvw[:, 2]).max())
+    f = make_cf_facet(0.0, 0.0, npix, cell_as, freqs, 8, [7] * 8, 11, wmax)
+    bmap = bdamod.bda_map_per_visibility(obs.a0, obs.a1, nchan)
+    chan_map = np.zeros(nchan, np.int32)
+    log("cfg1 built in %.1fs: %d rows x %d chan" % (time.time() - t0, R, nchan))
+    P = Problem(uvw=obs.uvw, vis=vis, flags=flags, weights=weights, freqs=freqs, a0=obs.a0, a1=obs.a1,
+                chan_map=chan_map, chan_map_degrid=chan_map.copy(), bda_grid=bmap, bda_degrid=bmap,
+                facets=[f], npix=npix, nband=1, npol=1, incr=np.array([-cell, cell]),
+                corr_codes=[9, 10, 11, 12], stokes_codes=[1])
+    P.extra.update(sky=sky, cell_arcsec=cell_as, image_npix=npix, name="cfg1")
+    return P
+
+
+def cfg2(nfx=9, image_npix=4096, ntime=600, nchan=64, nant=64, sort_by_baseline=True,
+         with_jones=True, with_vis=True, device="cpu", log=print, time_frac=1.0):
+    """The 81-facet MeerKAT-like workload.  `time_frac` < 1 keeps the first fraction of the
+    timesteps (for bounded CPU samples); everything else is unchanged."""
+    t0 = time.time()
+    freqs = np.linspace(0.9e9, 1.67e9, nchan)
+    nt = max(1, int(round(ntime * time_frac)))
+    obs = synth.make_observation("meerkat", nant, nt, 8.0, freqs, -30.0, -30.7, 2000,
+                                 sort_by_baseline=sort_by_baseline)
+    # time steps of the full observation, so a sample is a prefix of the full run
+    if nt != ntime:
+        full = synth.make_observation("meerkat", nant, ntime, 8.0, freqs, -30.0, -30.7, 2000)
+        obs = synth.make_observation("meerkat", nant, ntime, 8.0, freqs, -30.0, -30.7, 2000,
+                                     sort_by_baseline=sort_by_baseline)
+        keep = obs.tidx < nt
+        obs.uvw, obs.a0, obs.a1, obs.tidx = (np.ascontiguousarray(obs.uvw[keep]), obs.a0[keep].copy(),
+                                             obs.a1[keep].copy(), obs.tidx[keep].copy())
+        del full
+    R = obs.nrow
+    cell_as = 1.5
+    cell = cell_as * synth.ARCSEC
+    facet_px = image_npix / nfx
+    nopt, npad = estimate_npix(facet_px, 1.7)
+    log("cfg2: %d rows x %d chan, facets %d -> %d padded" % (R, nchan, nopt, npad))
+    cen = synth.facet_centres(nfx, nopt, cell)
+    wmax = float(np.abs(obs.uvw[:, 2]).max())
+    sups = [s if s % 2 else s + 1 for s in np.int64(np.linspace(7, 15, 32))]
+    facets = []
+    for i in range(nfx * nfx):
+        f = make_cf_facet(float(cen[i, 0]), float(cen[i, 1]), npad, cell_as, freqs, 32, sups, 11, wmax)
+        f.kms_dir = i
+        facets.append(f)
+    log("cfg2: CF tables in %.1fs" % (time.time() - t0))
+    chan_map = np.zeros(nchan, np.int32)
+    bmap = bdamod.bda_map_mode1(obs.uvw, obs.a0, obs.a1, freqs, obs.dfreq, 0.02, cell, nopt, chan_map)
+    log("cfg2: BDA map (%d blocks, %d words) in %.1fs" % (bdamod.nblocks_of(bmap), bmap.size, time.time() - t0))
+    jones = None
+    if with_jones:
+        ntj = 8
+        tsec = obs.tidx * 8.0
+        tmap = np.minimum((tsec // 600.0).astype(np.int32), ntj - 1)
+        cmap = (np.arange(nchan) // 8).astype(np.int32)
+        jones = dict(mode=1, interp=0, apply_amp=1, apply_phase=1, scale=0, calib_error=0.0,
+                     kms=synth.random_jones(ntj, nfx * nfx, nant, nchan // 8, 2004),
+                     kms_time=tmap, kms_chan=cmap)
+    rng = np.random.default_rng(2001)
+    sky = synth.sky_powerlaw(200, 0.5 * image_npix * cell, rng)
+    vis = None
+    if with_vis:
+        import torch
+        v = synth.predict_point_sources(obs.uvw, freqs, sky, device=device, chunk_rows=4096)
+        g = torch.Generator(device=device).manual_seed(2002)
+        noise = torch.randn(v.shape, generator=g, device=device, dtype=torch.complex64) * 0.01
+        v += noise
+        vis = v if device != "cpu" else v.numpy()
+        log("cfg2: visibilities in %.1fs" % (time.time() - t0))
+    flags = synth.random_flags(R, nchan, 4, 0.01, 2003)
+    weights = np.ones((R, nchan), np.float32)
+    P = Problem(uvw=obs.uvw, vis=vis, flags=flags, weights=weights, freqs=freqs, a0=obs.a0, a1=obs.a1,
+                chan_map=chan_map, chan_map_degrid=chan_map.copy(), bda_grid=bmap, bda_degrid=bmap,
+                facets=facets, npix=npad, nband=1, npol=1, incr=np.array([-cell, cell]),
+                corr_codes=[9, 10, 11, 12], stokes_codes=[1], jones=jones)
+    P.extra.update(sky=sky, cell_arcsec=cell_as, image_npix=image_npix, facet_npix=nopt, name="cfg2",
+                   nfx=nfx, equidistant=chan_equidistant(freqs))
+    return P

@@ -1,0 +1,136 @@
+// Definitions shared by the plain (ddfgpu.hip) and batched (ddf_batched.hip) translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ddf_device.h"
+
+namespace ddf {
+
+// Visibility-side kernel arguments (device pointers).
+struct VisP {
+  float2* vis;
+  const double* uvw;
+  const uint8_t* flags;
+  const float* weights;
+  const double* freqs;
+  const int32_t* chan_map;
+  int64_t nrow;
+  int nchan, ncorr;
+  const int32_t* bda;
+  const int64_t* bda_start;
+  int64_t nblocks, bda_len;
+  const uint8_t* sparse;
+};
+
+struct BlockRec {
+  int ch0, ch1, nr;
+  const int32_t* rows;
+};
+
+// Block b of the BDA map, with its channel range clamped to [0, nchan] and its row count to the
+// map length, so a malformed map cannot send a kernel outside its buffers.
+__device__ __forceinline__ BlockRec read_block(const VisP& v, int64_t b) {
+  int64_t s = v.bda_start[b];
+  BlockRec r;
+  int nr = v.bda[2 + b] - 2;
+  if (s < 0 || s + 2 > v.bda_len) {
+    r.ch0 = r.ch1 = 0;
+    r.nr = 0;
+    r.rows = v.bda;
+    return r;
+  }
+  if (nr < 0) nr = 0;
+  if (s + 2 + nr > v.bda_len) nr = int(v.bda_len - s - 2);
+  r.ch0 = clampi(v.bda[s], 0, v.nchan);
+  r.ch1 = clampi(v.bda[s + 1], r.ch0, v.nchan);
+  r.nr = nr;
+  r.rows = v.bda + s + 2;
+  return r;
+}
+
+// Phasor exp(i * freq * angle) at three precisions:
+//   0 mixed: FP64 phase and range reduction, FP32 sincos (~1e-7 for any phase);
+//   1 FP64 throughout;
+//   2 FP32 throughout (phase formed and reduced in FP32: the precision-study variant).
+template <int PREC>
+__device__ __forceinline__ cd phasor_prec(double freq, double angle) {
+  if (PREC == 1) {
+    double s, c;
+    sincos(freq * angle, &s, &c);
+    return cd{c, s};
+  } else if (PREC == 2) {
+    float s, c;
+    sincosf(float(freq) * float(angle), &s, &c);
+    return cd{double(c), double(s)};
+  } else {
+    return phasor(freq * angle);
+  }
+}
+
+// One facet as the batched kernels see it.
+struct FacetDev {
+  void* grid;              // complex64 [nband][npol][ny][nx] (gridding output)
+  const void* model;       // complex64 model grid (degridding input)
+  const float2* cf;        // CF planes, put order (Wplanes then WplanesConj)
+  const int64_t* cf_offset;// [2*nw]
+  double cu, cv, l0, m0, n0;
+  double dec_l0, dec_m0;   // LSmearing[5:7]
+  int kms_dir, beam_dir;
+  double* sumwt;           // [nband][npol]
+  double* sum_jones;       // [2][nband]
+  double* sum_jones_chan;  // [2][nchan]
+};
+
+struct BinItem {
+  uint32_t begin, end;     // record range in the binned record list
+  int16_t facet, band, tx, ty;
+};
+
+struct BatchParams {
+  const FacetDev* facets;
+  int nf;
+  int nband, npol, ny, nx;
+  double incr0, incr1;
+  int dopsf;
+  // CF geometry shared by every facet
+  const int32_t* support;  // [nw]
+  int nw, overs;
+  double ref_wave, wmax;
+  // Stokes map (grid: stokes <- corr; degrid: corr <- stokes)
+  int nstokes, read;
+  double sm[4][4][2];
+  // Jones tables (direction taken per facet) and decorrelation (l0, m0 per facet)
+  JonesP jp;
+  int jmode;
+  DecorrP dp;
+  double fmean0, wl_mean;
+  int prec;
+  // pass A layout
+  int group, slots, ngroups;
+  int64_t blocks_per_item, nchunks;
+  uint32_t nitems_a;
+  int chan_lds;
+  // tiles
+  int tile, halo, ntiles_x, ntiles_y;
+  uint32_t nitems_b;
+};
+
+// host launchers (ddf_batched.hip)
+int batched_place(const BatchParams& p, const VisP& v, bool degrid, uint64_t* place, hipStream_t st);
+int batched_count(const BatchParams& p, int64_t nblocks, const uint64_t* place, uint32_t* counts, hipStream_t st);
+int batched_fill(const BatchParams& p, int64_t nblocks, const uint64_t* place, const int64_t* bin_start,
+                 uint32_t* fill, uint32_t* recs, hipStream_t st);
+int batched_vis_counts(const VisP& v, int32_t* cnt, hipStream_t st);
+int batched_vlist(const VisP& v, const int64_t* vis_start, uint64_t* vrc, uint32_t* vblk, hipStream_t st);
+int batched_reset(uint32_t* counters, int n, hipStream_t st);
+int batched_grid_a(const BatchParams& p, const VisP& v, const uint64_t* place, float2* V, uint32_t* queue,
+                   unsigned nwg, hipStream_t st);
+int batched_grid_b(const BatchParams& p, const uint64_t* place, const float2* V, const BinItem* items,
+                   const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st);
+int batched_degrid_a(const BatchParams& p, const uint64_t* place, float2* M, const BinItem* items,
+                     const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st);
+int batched_degrid_b(const BatchParams& p, const VisP& v, const float2* M, const uint64_t* vrc, const uint32_t* vblk,
+                     int64_t nvl, hipStream_t st);
+
+}  // namespace ddf

@@ -18,10 +18,13 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "../../include/ddfgpu.h"
 #include "ddf_device.h"
+#include "ddf_internal.h"
 
 using namespace ddf;
 
@@ -143,21 +146,6 @@ int degrid_map(const ddf_vis* v, StokesMap* sm) {
 }
 
 // ---- kernel parameters -----------------------------------------------------------------------
-struct VisP {
-  float2* vis;
-  const double* uvw;
-  const uint8_t* flags;
-  const float* weights;
-  const double* freqs;
-  const int32_t* chan_map;
-  int64_t nrow;
-  int nchan, ncorr;
-  const int32_t* bda;
-  const int64_t* bda_start;
-  int64_t nblocks, bda_len;
-  const uint8_t* sparse;
-};
-
 struct KParams {
   VisP v;
   void* grid;
@@ -176,25 +164,6 @@ struct KParams {
 };
 
 __device__ __forceinline__ cd smc(const KParams& p, int r, int c) { return cd{p.sm[r][c][0], p.sm[r][c][1]}; }
-
-struct BlockRec {
-  int ch0, ch1, nr;
-  const int32_t* rows;
-};
-
-__device__ __forceinline__ BlockRec read_block(const VisP& v, int64_t b) {
-  int64_t s = v.bda_start[b];
-  BlockRec r;
-  int nr = v.bda[2 + b] - 2;
-  if (s < 0 || s + 2 > v.bda_len) { r.ch0 = r.ch1 = 0; r.nr = 0; r.rows = v.bda; return r; }
-  if (nr < 0) nr = 0;
-  if (s + 2 + nr > v.bda_len) nr = int(v.bda_len - s - 2);
-  r.ch0 = clampi(v.bda[s], 0, v.nchan);
-  r.ch1 = clampi(v.bda[s + 1], r.ch0, v.nchan);
-  r.nr = nr;
-  r.rows = v.bda + s + 2;
-  return r;
-}
 
 // ---- gridding kernel -----------------------------------------------------------------------
 // Block means over the unflagged (row, channel) visibilities (gridder.h:272-289): lanes take the
@@ -602,14 +571,19 @@ __global__ __launch_bounds__(256) void scan_chunk_final(const int32_t* sizes, in
   }
 }
 
-int launch_bda_index(const int32_t* bda, int64_t nblocks, int64_t* out, int64_t* chunk_tmp, hipStream_t st) {
-  if (nblocks <= 0) return DDF_OK;
-  int64_t nchunk = (nblocks + kScanChunk - 1) / kScanChunk;
-  hipLaunchKernelGGL(scan_chunk_sums, dim3(unsigned(nchunk)), dim3(256), 0, st, bda + 2, nblocks, chunk_tmp);
-  hipLaunchKernelGGL(scan_chunk_offsets, dim3(1), dim3(256), 0, st, chunk_tmp, nchunk, 2 + nblocks);
-  hipLaunchKernelGGL(scan_chunk_final, dim3(unsigned(nchunk)), dim3(256), 0, st, bda + 2, nblocks, chunk_tmp, out);
+// out[i] = base + sum_{j < i} x[j] (int32 in, int64 out); tmp holds ceil(n / kScanChunk) words.
+int launch_scan(const int32_t* x, int64_t n, int64_t base, int64_t* out, int64_t* chunk_tmp, hipStream_t st) {
+  if (n <= 0) return DDF_OK;
+  int64_t nchunk = (n + kScanChunk - 1) / kScanChunk;
+  hipLaunchKernelGGL(scan_chunk_sums, dim3(unsigned(nchunk)), dim3(256), 0, st, x, n, chunk_tmp);
+  hipLaunchKernelGGL(scan_chunk_offsets, dim3(1), dim3(256), 0, st, chunk_tmp, nchunk, base);
+  hipLaunchKernelGGL(scan_chunk_final, dim3(unsigned(nchunk)), dim3(256), 0, st, x, n, chunk_tmp, out);
   HIP_TRY(hipGetLastError());
   return DDF_OK;
+}
+
+int launch_bda_index(const int32_t* bda, int64_t nblocks, int64_t* out, int64_t* chunk_tmp, hipStream_t st) {
+  return launch_scan(bda + 2, nblocks, 2 + nblocks, out, chunk_tmp, st);
 }
 
 // ---- host-side assembly ---------------------------------------------------------------------
@@ -643,7 +617,7 @@ int check_common(const ddf_grid_desc* g, const ddf_vis* v, const ddf_facet* f) {
 }
 
 void fill_params(KParams& p, const ddf_grid_desc* g, const ddf_vis* v, const ddf_facet* f, const StokesMap& sm,
-                 const int64_t* bda_start, const std::vector<double>& freqs_host) {
+                 const int64_t* bda_start, double fmean0, double wl_mean) {
   std::memset(&p, 0, sizeof(p));
   p.v.vis = reinterpret_cast<float2*>(v->vis);
   p.v.uvw = v->uvw;
@@ -672,13 +646,8 @@ void fill_params(KParams& p, const ddf_grid_desc* g, const ddf_vis* v, const ddf
   p.l0 = f->l0;
   p.m0 = f->m0;
   p.n0 = std::sqrt(1 - f->l0 * f->l0 - f->m0 * f->m0) - 1;
-  double wl = 0, fm = 0;
-  for (double x : freqs_host) {
-    wl += kC / x;
-    fm += x;
-  }
-  p.wl_mean = wl / double(freqs_host.size());
-  p.fmean0 = fm / double(freqs_host.size());
+  p.wl_mean = wl_mean;
+  p.fmean0 = fmean0;
   p.cf.data = reinterpret_cast<const float2*>(f->cf.data);
   p.cf.offset = f->cf.offset;
   p.cf.support = f->cf.support;
@@ -759,10 +728,24 @@ int resolve_block_index(const ddf_vis* v, Temp& tmp, const int64_t** out) {
   return rc;
 }
 
-int host_freqs(const ddf_vis* v, std::vector<double>& f, hipStream_t st) {
-  f.resize(size_t(v->nchan));
+// Mean frequency and wavelength of the channels (gridder.h:164-171): taken from the host fields
+// when set, else read back from the device array (a synchronising copy).
+int freq_means(const ddf_vis* v, double* fmean, double* wlmean, hipStream_t st) {
+  if (v->freq_mean > 0 && v->wavelength_mean > 0) {
+    *fmean = v->freq_mean;
+    *wlmean = v->wavelength_mean;
+    return DDF_OK;
+  }
+  std::vector<double> f(size_t(v->nchan));
   HIP_TRY(hipMemcpyAsync(f.data(), v->freqs, f.size() * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
+  double wl = 0, fm = 0;
+  for (double x : f) {
+    wl += kC / x;
+    fm += x;
+  }
+  *fmean = fm / double(f.size());
+  *wlmean = wl / double(f.size());
   return DDF_OK;
 }
 
@@ -799,6 +782,10 @@ int launch_degrid_r(const KParams& p, int jmode, unsigned nb, hipStream_t st) {
 
 }  // namespace
 
+namespace ddf {
+void set_last_error(const std::string& msg) { g_err = msg; }
+}  // namespace ddf
+
 extern "C" {
 
 int ddf_version(void) { return DDF_ABI_VERSION; }
@@ -832,15 +819,15 @@ int ddf_grid(const ddf_grid_desc* g, const ddf_vis* v, const ddf_facet* f, void*
   if (rc) return rc;
   if (v->nblocks == 0 || v->nrow == 0) return DDF_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  std::vector<double> fh;
-  rc = host_freqs(v, fh, st);
+  double fmean0 = 0, wl_mean = 0;
+  rc = freq_means(v, &fmean0, &wl_mean, st);
   if (rc) return rc;
   Temp tmp(st);
   const int64_t* starts = nullptr;
   rc = resolve_block_index(v, tmp, &starts);
   if (rc) return rc;
   KParams p;
-  fill_params(p, g, v, f, sm, starts, fh);
+  fill_params(p, g, v, f, sm, starts, fmean0, wl_mean);
   const int jmode = f->jones.enabled ? f->jones.mode : 0;
   size_t lds_base = size_t(g->nband) * g->npol + 2 * size_t(g->nband);
   p.chan_lds = (jmode != 0) && (lds_base + 2 * size_t(v->nchan)) * 8 <= 48 * 1024;
@@ -865,15 +852,15 @@ int ddf_degrid(const ddf_grid_desc* g, const ddf_vis* v, const ddf_facet* f, voi
   if (sm.nstokes > g->npol) return fail(DDF_EINVAL, "grid has fewer polarisations than the requested Stokes parameters");
   if (v->nblocks == 0 || v->nrow == 0) return DDF_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  std::vector<double> fh;
-  rc = host_freqs(v, fh, st);
+  double fmean0 = 0, wl_mean = 0;
+  rc = freq_means(v, &fmean0, &wl_mean, st);
   if (rc) return rc;
   Temp tmp(st);
   const int64_t* starts = nullptr;
   rc = resolve_block_index(v, tmp, &starts);
   if (rc) return rc;
   KParams p;
-  fill_params(p, g, v, f, sm, starts, fh);
+  fill_params(p, g, v, f, sm, starts, fmean0, wl_mean);
   const int jmode = f->jones.enabled ? f->jones.mode : 0;
   unsigned nb = grid_blocks(v->nblocks);
   if (v->ncorr == 4) rc = launch_degrid_r<4>(p, jmode, nb, st);
@@ -911,5 +898,414 @@ extern "C" int ddf_accumulate_weights(double* grid, int64_t grid_len, const floa
   hipLaunchKernelGGL(accumulate_weights, dim3(unsigned(nb)), dim3(256), 0, static_cast<hipStream_t>(stream), grid,
                      grid_len, weights, index, n);
   HIP_TRY(hipGetLastError());
+  return DDF_OK;
+}
+
+// ======================================================================================================
+// Batched plan (host side).
+// ======================================================================================================
+struct ddf_plan {
+  BatchParams pg, pd;
+  VisP vg, vd;
+  FacetDev* d_facets = nullptr;
+  int32_t* d_support = nullptr;
+  uint64_t* place_g = nullptr;
+  uint64_t* place_d = nullptr;
+  uint32_t* recs_g = nullptr;
+  uint32_t* recs_d = nullptr;
+  BinItem* items_g = nullptr;
+  BinItem* items_d = nullptr;
+  float2* V = nullptr;
+  float2* M = nullptr;
+  uint64_t* vrc = nullptr;
+  uint32_t* vblk = nullptr;
+  int64_t nvl = 0;
+  uint32_t* queues = nullptr;
+  unsigned nwg = 0;
+  int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  std::vector<void*> allocs;
+  ~ddf_plan() {
+    for (void* p : allocs) (void)hipFree(p);
+  }
+  template <typename T>
+  T* alloc(size_t n) {
+    void* p = nullptr;
+    size_t bytes = n * sizeof(T);
+    if (bytes == 0) bytes = sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    allocs.push_back(p);
+    stats[5] += int64_t(bytes);
+    return static_cast<T*>(p);
+  }
+};
+
+namespace {
+
+int fill_vis(VisP& out, const ddf_vis* v, const int64_t* starts) {
+  std::memset(&out, 0, sizeof(out));
+  out.vis = reinterpret_cast<float2*>(v->vis);
+  out.uvw = v->uvw;
+  out.flags = v->flags;
+  out.weights = v->weights;
+  out.freqs = v->freqs;
+  out.chan_map = v->chan_map;
+  out.nrow = v->nrow;
+  out.nchan = v->nchan;
+  out.ncorr = v->ncorr;
+  out.bda = v->bda;
+  out.bda_start = starts;
+  out.nblocks = v->nblocks;
+  out.bda_len = v->bda_len;
+  out.sparse = v->sparsification;
+  return DDF_OK;
+}
+
+void fill_batch(BatchParams& p, const ddf_grid_desc* g, const ddf_facet* f0, const StokesMap& sm,
+                double fmean0, double wl_mean) {
+  std::memset(&p, 0, sizeof(p));
+  p.nband = g->nband;
+  p.npol = g->npol;
+  p.ny = g->ny;
+  p.nx = g->nx;
+  p.incr0 = g->incr[0];
+  p.incr1 = g->incr[1];
+  p.nw = f0->cf.nw;
+  p.overs = f0->cf.overs;
+  p.ref_wave = f0->cf.ref_wave;
+  p.wmax = f0->cf.wmax;
+  p.nstokes = sm.nstokes;
+  p.read = sm.read;
+  std::memcpy(p.sm, sm.m, sizeof(p.sm));
+  const ddf_jones& j = f0->jones;
+  p.jmode = j.enabled ? j.mode : 0;
+  p.jp.enabled = j.enabled;
+  p.jp.mode = j.mode;
+  p.jp.interp = j.interp_mode;
+  p.jp.kms = reinterpret_cast<const float2*>(j.kms);
+  p.jp.beam = reinterpret_cast<const float2*>(j.beam);
+  for (int i = 0; i < 4; ++i) {
+    p.jp.kdims[i] = j.kms_dims[i];
+    p.jp.bdims[i] = j.beam_dims[i];
+  }
+  p.jp.ktime = j.kms_time;
+  p.jp.kchan = j.kms_chan;
+  p.jp.btime = j.beam_time;
+  p.jp.bchan = j.beam_chan;
+  p.jp.a0 = j.a0;
+  p.jp.a1 = j.a1;
+  p.jp.coefs = j.coefs;
+  p.jp.ncoefs = j.coefs ? j.ncoefs : 0;
+  p.jp.apply_amp = j.apply_amp;
+  p.jp.apply_phase = j.apply_phase;
+  p.jp.scale = j.scale_jones;
+  p.jp.calib_error = j.calib_error;
+  p.dp.enabled = f0->decorr.enabled;
+  p.dp.tsmear = f0->decorr.tsmear;
+  p.dp.fsmear = f0->decorr.fsmear;
+  p.dp.uvw_dt = f0->decorr.uvw_dt;
+  p.dp.dt = f0->decorr.dt;
+  p.dp.dnu = f0->decorr.dnu;
+  p.fmean0 = fmean0;
+  p.wl_mean = wl_mean;
+}
+
+// Count -> scan -> fill of the valid records, then the work items (largest first).
+int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place, int chunk, uint32_t** recs_out,
+               BinItem** items_out, uint32_t* nitems, int64_t* nrec, hipStream_t st) {
+  const int64_t nbins = int64_t(p.nf) * p.nband * p.ntiles_y * p.ntiles_x;
+  uint32_t* counts = P->alloc<uint32_t>(size_t(nbins));
+  uint32_t* fill = P->alloc<uint32_t>(size_t(nbins));
+  int64_t* bstart = P->alloc<int64_t>(size_t(nbins));
+  if (!counts || !fill || !bstart) return fail(DDF_ENOMEM, "plan allocation failed");
+  HIP_TRY(hipMemsetAsync(counts, 0, size_t(nbins) * 4, st));
+  HIP_TRY(hipMemsetAsync(fill, 0, size_t(nbins) * 4, st));
+  if (batched_count(p, v.nblocks, place, counts, st)) return fail(DDF_EHIP, "plan_count launch failed");
+  std::vector<uint32_t> hc(static_cast<size_t>(nbins));
+  HIP_TRY(hipMemcpyAsync(hc.data(), counts, size_t(nbins) * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  std::vector<int64_t> hs(static_cast<size_t>(nbins));
+  int64_t tot = 0;
+  std::vector<BinItem> items;
+  for (int64_t i = 0; i < nbins; ++i) {
+    hs[size_t(i)] = tot;
+    const int64_t c = hc[size_t(i)];
+    if (c) {
+      int64_t rem = i;
+      const int tx = int(rem % p.ntiles_x); rem /= p.ntiles_x;
+      const int ty = int(rem % p.ntiles_y); rem /= p.ntiles_y;
+      const int band = int(rem % p.nband); rem /= p.nband;
+      const int f = int(rem);
+      for (int64_t b = 0; b < c; b += chunk) {
+        BinItem it;
+        it.begin = uint32_t(tot + b);
+        it.end = uint32_t(tot + (b + chunk < c ? b + chunk : c));
+        it.facet = int16_t(f);
+        it.band = int16_t(band);
+        it.tx = int16_t(tx);
+        it.ty = int16_t(ty);
+        items.push_back(it);
+      }
+    }
+    tot += c;
+  }
+  if (tot >= (int64_t(1) << 32)) return fail(DDF_EINVAL, "too many (block, facet) records for one plan");
+  std::stable_sort(items.begin(), items.end(),
+                   [](const BinItem& a, const BinItem& b) { return (a.end - a.begin) > (b.end - b.begin); });
+  HIP_TRY(hipMemcpyAsync(bstart, hs.data(), size_t(nbins) * 8, hipMemcpyHostToDevice, st));
+  uint32_t* recs = P->alloc<uint32_t>(size_t(tot));
+  BinItem* d_items = P->alloc<BinItem>(items.size());
+  if (!recs || !d_items) return fail(DDF_ENOMEM, "plan allocation failed");
+  if (!items.empty())
+    HIP_TRY(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(BinItem), hipMemcpyHostToDevice, st));
+  if (batched_fill(p, v.nblocks, place, bstart, fill, recs, st)) return fail(DDF_EHIP, "plan_fill launch failed");
+  HIP_TRY(hipStreamSynchronize(st));
+  *recs_out = recs;
+  *items_out = d_items;
+  *nitems = uint32_t(items.size());
+  *nrec = tot;
+  return DDF_OK;
+}
+
+}  // namespace
+
+extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, const ddf_vis* vdegrid,
+                               const ddf_facet* facets, void* const* models, int32_t nfacets,
+                               const ddf_plan_opts* opts, ddf_plan** out, void* stream) {
+  if (!out) return fail(DDF_EINVAL, "null plan output");
+  *out = nullptr;
+  if (nfacets <= 0 || !facets || !models) return fail(DDF_EINVAL, "no facets");
+  int rc = check_common(g, vgrid, &facets[0]);
+  if (rc) return rc;
+  rc = check_common(g, vdegrid, &facets[0]);
+  if (rc) return rc;
+  if (g->grid_f64) return fail(DDF_EINVAL, "the batched path accumulates complex64 grids");
+  if (g->nx >= 65536 || g->ny >= 65536 || g->nband >= 128) return fail(DDF_EINVAL, "grid too large for the plan records");
+  if (!vgrid->weights) return fail(DDF_EINVAL, "weights pointer is null");
+  const ddf_cf& c0 = facets[0].cf;
+  if (2 * c0.nw > 255 || c0.overs > 255) return fail(DDF_EINVAL, "too many w-planes / oversampling for the plan records");
+  StokesMap smg, smd;
+  rc = grid_map(vgrid, &smg);
+  if (rc) return rc;
+  rc = degrid_map(vdegrid, &smd);
+  if (rc) return rc;
+  if (smd.nstokes > g->npol) return fail(DDF_EINVAL, "grid has fewer polarisations than the requested Stokes parameters");
+  if ((vgrid->nblocks + 1) * int64_t(nfacets) >= (int64_t(1) << 32) ||
+      (vdegrid->nblocks + 1) * int64_t(nfacets) >= (int64_t(1) << 32))
+    return fail(DDF_EINVAL, "blocks x facets exceeds the 32-bit record index");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // shared CF geometry
+  std::vector<int32_t> sup0(static_cast<size_t>(c0.nw)), supi(static_cast<size_t>(c0.nw));
+  HIP_TRY(hipMemcpyAsync(sup0.data(), c0.support, sup0.size() * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  int maxsup = 0;
+  for (int32_t x : sup0) {
+    if (x <= 0 || x % 2 == 0) return fail(DDF_EINVAL, "CF supports must be odd and positive");
+    maxsup = x > maxsup ? x : maxsup;
+  }
+  for (int i = 1; i < nfacets; ++i) {
+    const ddf_cf& ci = facets[i].cf;
+    if (ci.nw != c0.nw || ci.overs != c0.overs || ci.ref_wave != c0.ref_wave || ci.wmax != c0.wmax)
+      return fail(DDF_EINVAL, "all facets of a plan must share nw, OverS, RefWave and wmax");
+    HIP_TRY(hipMemcpyAsync(supi.data(), ci.support, supi.size() * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (supi != sup0) return fail(DDF_EINVAL, "all facets of a plan must share the CF supports");
+    if (!facets[i].grid || !models[i] || !facets[i].sumwt) return fail(DDF_EINVAL, "facet buffer is null");
+    if (facets[0].jones.enabled && (!facets[i].jones.sum_jones || !facets[i].jones.sum_jones_chan))
+      return fail(DDF_EINVAL, "SumJones buffers are null");
+  }
+  if (!models[0] || !facets[0].sumwt) return fail(DDF_EINVAL, "facet buffer is null");
+  if (facets[0].jones.enabled && (!facets[0].jones.sum_jones || !facets[0].jones.sum_jones_chan))
+    return fail(DDF_EINVAL, "SumJones buffers are null");
+
+  ddf_plan* P = new ddf_plan();
+  auto bail = [&](int code) {
+    delete P;
+    return code;
+  };
+  double fm = 0, wl = 0;
+  rc = freq_means(vgrid, &fm, &wl, st);
+  if (rc) return bail(rc);
+  fill_batch(P->pg, g, &facets[0], smg, fm, wl);
+  fill_batch(P->pd, g, &facets[0], smd, fm, wl);
+  P->pg.dopsf = g->dopsf;
+  ddf_plan_opts o{};
+  if (opts) o = *opts;
+  const int tile = o.tile > 0 ? o.tile : 32;
+  const int chunk = o.chunk > 0 ? o.chunk : 2048;
+  const int halo = (maxsup - 1) / 2;
+  int G = o.group;
+  if (G <= 0) {
+    int ng = (nfacets + 31) / 32;
+    G = (nfacets + ng - 1) / ng;
+  }
+  if (G > 256) G = 256;
+  // facet descriptors
+  std::vector<FacetDev> hf(static_cast<size_t>(nfacets));
+  for (int i = 0; i < nfacets; ++i) {
+    const ddf_facet& f = facets[i];
+    FacetDev& d = hf[size_t(i)];
+    d.grid = f.grid;
+    d.model = models[i];
+    d.cf = reinterpret_cast<const float2*>(f.cf.data);
+    d.cf_offset = f.cf.offset;
+    d.cu = f.cu;
+    d.cv = f.cv;
+    d.l0 = f.l0;
+    d.m0 = f.m0;
+    d.n0 = std::sqrt(1 - f.l0 * f.l0 - f.m0 * f.m0) - 1;
+    d.dec_l0 = f.decorr.l0;
+    d.dec_m0 = f.decorr.m0;
+    d.kms_dir = f.jones.kms_dir;
+    d.beam_dir = f.jones.beam_dir;
+    d.sumwt = f.sumwt;
+    d.sum_jones = f.jones.sum_jones;
+    d.sum_jones_chan = f.jones.sum_jones_chan;
+  }
+  P->d_facets = P->alloc<FacetDev>(hf.size());
+  P->d_support = P->alloc<int32_t>(sup0.size());
+  P->queues = P->alloc<uint32_t>(4);
+  if (!P->d_facets || !P->d_support || !P->queues) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
+  if (hipMemcpyAsync(P->d_facets, hf.data(), hf.size() * sizeof(FacetDev), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(P->d_support, sup0.data(), sup0.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+    return bail(fail(DDF_EHIP, "plan upload failed"));
+  for (BatchParams* bp : {&P->pg, &P->pd}) {
+    bp->facets = P->d_facets;
+    bp->nf = nfacets;
+    bp->support = P->d_support;
+    bp->prec = o.precision;
+    bp->tile = tile;
+    bp->halo = halo;
+    bp->ntiles_x = (g->nx + tile - 1) / tile;
+    bp->ntiles_y = (g->ny + tile - 1) / tile;
+    bp->group = G;
+    bp->slots = 256 / G;
+    bp->ngroups = (nfacets + G - 1) / G;
+  }
+  P->pd.jmode = vdegrid == vgrid ? P->pg.jmode : P->pd.jmode;
+  // block indices of both maps
+  const int64_t* sg = vgrid->bda_start;
+  const int64_t* sd = vdegrid->bda_start;
+  for (int k = 0; k < 2; ++k) {
+    const ddf_vis* v = k ? vdegrid : vgrid;
+    const int64_t*& s = k ? sd : sg;
+    if (s || v->nblocks == 0) continue;
+    int64_t* ns = P->alloc<int64_t>(size_t(v->nblocks));
+    int64_t* tmp = P->alloc<int64_t>(size_t((v->nblocks + kScanChunk - 1) / kScanChunk));
+    if (!ns || !tmp) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
+    rc = launch_bda_index(v->bda, v->nblocks, ns, tmp, st);
+    if (rc) return bail(rc);
+    s = ns;
+  }
+  fill_vis(P->vg, vgrid, sg);
+  fill_vis(P->vd, vdegrid, sd);
+  // placements
+  P->place_g = P->alloc<uint64_t>(size_t(vgrid->nblocks) * nfacets);
+  P->place_d = P->alloc<uint64_t>(size_t(vdegrid->nblocks) * nfacets);
+  if (!P->place_g || !P->place_d) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
+  if (vgrid->nblocks && batched_place(P->pg, P->vg, false, P->place_g, st)) return bail(fail(DDF_EHIP, "plan_place failed"));
+  if (vdegrid->nblocks && batched_place(P->pd, P->vd, true, P->place_d, st)) return bail(fail(DDF_EHIP, "plan_place failed"));
+  // bins
+  uint32_t ni = 0;
+  int64_t nr = 0;
+  rc = build_bins(P, P->pg, P->vg, P->place_g, chunk, &P->recs_g, &P->items_g, &ni, &nr, st);
+  if (rc) return bail(rc);
+  P->pg.nitems_b = ni;
+  P->stats[0] = nr;
+  P->stats[2] = ni;
+  rc = build_bins(P, P->pd, P->vd, P->place_d, chunk, &P->recs_d, &P->items_d, &ni, &nr, st);
+  if (rc) return bail(rc);
+  P->pd.nitems_b = ni;
+  P->stats[1] = nr;
+  P->stats[3] = ni;
+  // pass A layout
+  const int64_t bpi = o.blocks_per_item > 0 ? o.blocks_per_item : 128;
+  P->pg.blocks_per_item = bpi;
+  P->pg.nchunks = (vgrid->nblocks + bpi - 1) / bpi;
+  if (int64_t(P->pg.ngroups) * P->pg.nchunks >= (int64_t(1) << 32)) return bail(fail(DDF_EINVAL, "too many work items"));
+  P->pg.nitems_a = uint32_t(int64_t(P->pg.ngroups) * P->pg.nchunks);
+  const size_t per_f = size_t(g->nband) * g->npol + 2 * size_t(g->nband);
+  if (size_t(nfacets) * per_f * 8 > 40 * 1024) return bail(fail(DDF_EINVAL, "facets x bands too large for one plan"));
+  P->pg.chan_lds = (P->pg.jmode != 0) &&
+                   (size_t(nfacets) * per_f + size_t(G) * 2 * size_t(vgrid->nchan)) * 8 <= 64 * 1024;
+  // LDS tiles
+  const int W = tile + 2 * halo;
+  const int npg = smg.nstokes < g->npol ? smg.nstokes : g->npol;
+  if (size_t(npg > smd.nstokes ? npg : smd.nstokes) * W * W * 8 > 64 * 1024)
+    return bail(fail(DDF_EINVAL, "tile too large for LDS: lower opts.tile"));
+  // record values
+  P->V = P->alloc<float2>(size_t(vgrid->nblocks) * nfacets * smg.nstokes);
+  P->M = P->alloc<float2>(size_t(vdegrid->nblocks) * nfacets * smd.nstokes);
+  if (!P->V || !P->M) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
+  if (hipMemsetAsync(P->M, 0, size_t(vdegrid->nblocks) * nfacets * smd.nstokes * 8, st) != hipSuccess)
+    return bail(fail(DDF_EHIP, "memset failed"));
+  // visibility list of the degrid map
+  if (vdegrid->nblocks) {
+    int32_t* cnt = P->alloc<int32_t>(size_t(vdegrid->nblocks));
+    int64_t* vs = P->alloc<int64_t>(size_t(vdegrid->nblocks));
+    int64_t* tmp = P->alloc<int64_t>(size_t((vdegrid->nblocks + kScanChunk - 1) / kScanChunk));
+    if (!cnt || !vs || !tmp) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
+    if (batched_vis_counts(P->vd, cnt, st)) return bail(fail(DDF_EHIP, "launch failed"));
+    rc = launch_scan(cnt, vdegrid->nblocks, 0, vs, tmp, st);
+    if (rc) return bail(rc);
+    int64_t last_start = 0;
+    int32_t last_cnt = 0;
+    if (hipMemcpyAsync(&last_start, vs + vdegrid->nblocks - 1, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(&last_cnt, cnt + vdegrid->nblocks - 1, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return bail(fail(DDF_EHIP, "copy failed"));
+    P->nvl = last_start + last_cnt;
+    P->vrc = P->alloc<uint64_t>(size_t(P->nvl));
+    P->vblk = P->alloc<uint32_t>(size_t(P->nvl));
+    if (!P->vrc || !P->vblk) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
+    if (batched_vlist(P->vd, vs, P->vrc, P->vblk, st)) return bail(fail(DDF_EHIP, "launch failed"));
+  }
+  P->stats[4] = P->nvl;
+  P->stats[6] = tile;
+  P->stats[7] = G;
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return bail(fail(DDF_EHIP, "device query failed"));
+  P->nwg = unsigned(o.workgroups > 0 ? o.workgroups : prop.multiProcessorCount * 4);
+  if (hipStreamSynchronize(st) != hipSuccess) return bail(fail(DDF_EHIP, "plan build failed"));
+  *out = P;
+  return DDF_OK;
+}
+
+extern "C" int ddf_plan_grid(ddf_plan* P, float* vis, int32_t dopsf, void* stream) {
+  if (!P) return fail(DDF_EINVAL, "null plan");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  VisP v = P->vg;
+  if (vis) v.vis = reinterpret_cast<float2*>(vis);
+  BatchParams p = P->pg;
+  p.dopsf = dopsf;
+  if (batched_reset(P->queues, 2, st)) return fail(DDF_EHIP, "launch failed");
+  if (p.nitems_a && batched_grid_a(p, v, P->place_g, P->V, P->queues, P->nwg, st)) return fail(DDF_EHIP, "grid pass A launch failed");
+  if (p.nitems_b && batched_grid_b(p, P->place_g, P->V, P->items_g, P->recs_g, P->queues + 1, P->nwg, st))
+    return fail(DDF_EHIP, "grid pass B launch failed");
+  return DDF_OK;
+}
+
+extern "C" int ddf_plan_degrid(ddf_plan* P, float* vis, void* stream) {
+  if (!P) return fail(DDF_EINVAL, "null plan");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  VisP v = P->vd;
+  if (vis) v.vis = reinterpret_cast<float2*>(vis);
+  if (batched_reset(P->queues + 2, 1, st)) return fail(DDF_EHIP, "launch failed");
+  if (P->pd.nitems_b && batched_degrid_a(P->pd, P->place_d, P->M, P->items_d, P->recs_d, P->queues + 2, P->nwg, st))
+    return fail(DDF_EHIP, "degrid pass A launch failed");
+  if (P->nvl && batched_degrid_b(P->pd, v, P->M, P->vrc, P->vblk, P->nvl, st))
+    return fail(DDF_EHIP, "degrid pass B launch failed");
+  return DDF_OK;
+}
+
+extern "C" int ddf_plan_info(const ddf_plan* P, int64_t* out, int32_t n) {
+  if (!P || !out) return fail(DDF_EINVAL, "null argument");
+  for (int i = 0; i < n && i < 8; ++i) out[i] = P->stats[i];
+  return DDF_OK;
+}
+
+extern "C" int ddf_plan_destroy(ddf_plan* P) {
+  delete P;
   return DDF_OK;
 }

@@ -270,6 +270,16 @@ def parse_degrid_args(args):
     return CallSpec(grid=grid, grid_f64=0, vis=vis, weights=None, sumwt=None, **common)
 
 
+def freq_means(freqs):
+    """(mean frequency, mean wavelength) summed in channel order, as gridder.h:164-171 does."""
+    wl = 0.0
+    fm = 0.0
+    for f in np.asarray(freqs, dtype=np.float64):
+        wl += 299792458.0 / f
+        fm += f
+    return fm / len(freqs), wl / len(freqs)
+
+
 def fill_structs(spec, addr, bda_start=None):
     """Build (ddf_grid_desc, ddf_vis, ddf_facet) for `spec`; addr(name, ndarray) -> int address
     (0 for None).  The caller keeps whatever addr allocates alive for the call."""
@@ -299,6 +309,7 @@ def fill_structs(spec, addr, bda_start=None):
     v.nstokes = int(spec.stokes_codes.size)
     for i, c in enumerate(spec.stokes_codes):
         v.stokes_codes[i] = int(c)
+    v.freq_mean, v.wavelength_mean = freq_means(spec.freqs)
 
     f = abi.ddf_facet()
     f.grid = addr("grid", spec.grid)

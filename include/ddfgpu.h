@@ -112,6 +112,8 @@ typedef struct ddf_vis {
   int32_t  ncorr_codes;
   uint16_t stokes_codes[4]; /* ExpectedOutStokes                                                */
   int32_t  nstokes;
+  double   freq_mean;       /* host: mean of freqs[]      (0: computed from the device array)  */
+  double   wavelength_mean; /* host: mean of c / freqs[]  (0: computed from the device array)  */
 } ddf_vis;
 
 /* One facet: its grid and its geometry.                                                         */
@@ -154,6 +156,77 @@ int ddf_degrid(const ddf_grid_desc* g, const ddf_vis* v, const ddf_facet* f, voi
  * grid[index[i]] += weights[i] for weights[i] != 0.  grid float64, weights float32, index int64. */
 int ddf_accumulate_weights(double* grid, int64_t grid_len, const float* weights,
                            const int64_t* index, int64_t n, void* stream);
+
+/* ---- batched major cycle: every facet of a chunk in a few persistent launches ----------------
+ * Replaces the per-facet job loops of ClassFacetMachine (DDFacet/Imager/ClassFacetMachine.py:
+ * 1574-1592 gridding, 1912-1943 degridding, one APP job per facet) and the row-semaphore
+ * reduction of the degridder (degridder.h:236-263, Semaphores.cc:27-58).  No reference entry
+ * point corresponds one to one: a caller that used to loop pyGridderWPol32 / pyDeGridderWPol over
+ * the facets of a chunk builds one plan and calls ddf_plan_grid / ddf_plan_degrid per cycle.
+ *
+ * A plan depends on uvw, flags, frequencies, channel / BDA maps, CF geometry and facet geometry,
+ * not on visibility or model values: build it once per chunk and reuse it every major cycle.
+ * facets[i].cf lists the planes in put() order (Wplanes first); models[i] is facet i's complex64
+ * model grid for degridding; facets[i].grid receives facet i's gridded visibilities.  All facets
+ * share nw, overs, ref_wave, wmax and the per-plane supports (as ClassWTermModified builds them).
+ * The Jones tables are taken from facets[0].jones; each facet supplies its direction indices and
+ * its sum_jones / sum_jones_chan outputs.  vgrid / vdegrid carry the grid / degrid BDA maps.   */
+typedef struct ddf_plan ddf_plan;
+
+typedef struct ddf_plan_opts {
+  int32_t tile;            /* uv tile edge in grid cells (0: 32)                                */
+  int32_t chunk;           /* max records per tile work item (0: 2048)                          */
+  int32_t blocks_per_item; /* BDA blocks per gridding work item (0: 128)                        */
+  int32_t group;           /* facets per lane group in gridding (0: auto, <= 64)                */
+  int32_t precision;       /* 0 FP64 phase base + FP32 sincos + FP64 block sums; 1 FP64; 2 FP32 */
+  int32_t workgroups;      /* persistent workgroups per launch (0: 4 per compute unit)          */
+} ddf_plan_opts;
+
+int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, const ddf_vis* vdegrid,
+                    const ddf_facet* facets, void* const* models, int32_t nfacets,
+                    const ddf_plan_opts* opts, ddf_plan** plan, void* stream);
+/* Grid `vis` (NULL: vgrid->vis) onto every facet grid; accumulates grids, sumwt and the Jones
+ * sums exactly like one pyGridderWPol32 call per facet.                                         */
+int ddf_plan_grid(ddf_plan* plan, float* vis, int32_t dopsf, void* stream);
+/* vis (NULL: vdegrid->vis) -= sum over facets of the degridded models, like one
+ * pyDeGridderWPol call per facet.                                                                */
+int ddf_plan_degrid(ddf_plan* plan, float* vis, void* stream);
+/* Counters: [0] grid records, [1] degrid records, [2] grid work items, [3] degrid work items,
+ * [4] degrid visibilities, [5] device bytes held, [6] tile, [7] facet group.                    */
+int ddf_plan_info(const ddf_plan* plan, int64_t* out, int32_t n);
+int ddf_plan_destroy(ddf_plan* plan);
+
+/* ---- facet image side of the major cycle ---------------------------------------------------
+ * One facet of the image plane.  Device pointers; n = npix_facet (the padded facet size).      */
+typedef struct ddf_image_facet {
+  int32_t xc, yc;           /* DicoImager[i]["pixCentral"]: facet centre in main-image pixels    */
+  const float* sphe;        /* float32 [n][n] cf_dict["Sphe"]                                    */
+  const float* inv_sphe;    /* float32 [n][n] cf_dict["InvSphe"]                                 */
+  const float* sw;          /* float32 [n][n] cf_dict["SW"] (the stored spatial weight)          */
+  const float* dirty;       /* complex64 [nband][npol][n][n] facet image after GridToIm          */
+  float* model;             /* complex64 [nband][npol][n][n] model grid written by model_to_grid */
+  const double* sumwt;      /* float64 [nband][npol]                                             */
+  const double* sum_jones;  /* float64 [2][nband], or NULL without Jones                         */
+} ddf_image_facet;
+
+/* Centred 2-D FFT of nplanes n x n complex64 planes in place, ModFFTW.FFTW_2Donly
+ * (DDFacet/ToolsDir/ModFFTW.py:557-600): forward fftshift(fft2(ifftshift(A))) / n^2, inverse
+ * fftshift(ifft2(ifftshift(A))) * n^2, both times `scale`.  GridToIm
+ * (ClassDDEGridMachine.py:1186-1190) is ddf_fft2c(grid, n, nband*npol, 1, OverS^2).            */
+int ddf_fft2c(float* planes, int32_t n, int64_t nplanes, int32_t inverse, double scale, void* stream);
+/* BuildFacetNormImage (ClassFacetMachine.py:1289-1325): facet_norm float32 [m][m].              */
+int ddf_facet_norm(const ddf_image_facet* facets, int32_t nf, int32_t npix_facet, int32_t m,
+                   float* facet_norm, void* stream);
+/* FacetsToIm_Channel, kind Dirty / PSF (ClassFacetMachine.py:1331-1416): image float32
+ * [nband][npol][m][m] (overwritten).                                                             */
+int ddf_stitch(const ddf_image_facet* facets, int32_t nf, int32_t npix_facet, int32_t nband,
+               int32_t npol, int32_t m, const float* facet_norm, float* image, void* stream);
+/* GiveModelTessel(ToGrid=False, ApplyNorm=True) (ClassImToGrid.py:138-264) and the forward FFT of
+ * get(TranformModelInput="FT") (ClassDDEGridMachine.py:951-958): writes facets[i].model from
+ * the main model image float32 [nband][npol][m][m].                                               */
+int ddf_model_to_grid(const ddf_image_facet* facets, int32_t nf, int32_t npix_facet, int32_t nband,
+                      int32_t npol, int32_t m, const float* image, const float* facet_norm,
+                      int32_t overs, void* stream);
 
 #ifdef __cplusplus
 }
