@@ -164,6 +164,9 @@ EXTRA_SIGNATURES = {
     "ddf_plan_degrid": [C.c_void_p, C.c_void_p, C.c_void_p],
     "ddf_plan_info": [C.c_void_p, C.POINTER(C.c_int64), C.c_int32],
     "ddf_plan_destroy": [C.c_void_p],
+    "ddf_plan_degrid_adjoint": [C.c_void_p, C.c_void_p, C.c_void_p],
+    "ddf_model_to_grid_adjoint": [C.POINTER(ddf_image_facet), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                  C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p],
     "ddf_fft2c": [C.c_void_p, C.c_int32, C.c_int64, C.c_int32, C.c_double, C.c_void_p],
     "ddf_facet_norm": [C.POINTER(ddf_image_facet), C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p],
     "ddf_stitch": [C.POINTER(ddf_image_facet), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,

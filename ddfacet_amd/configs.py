@@ -141,3 +141,51 @@ def cfg2(nfx=9, image_npix=4096, ntime=600, nchan=64, nant=64, sort_by_baseline=
     P.extra.update(sky=sky, cell_arcsec=cell_as, image_npix=image_npix, facet_npix=nopt, name="cfg2",
                    nfx=nfx, equidistant=chan_equidistant(freqs))
     return P
+
+
+def small_cycle(seed=5, nfx=3, m=243, n=147, cell_arcsec=20.0, nant=7, ntime=10, nchan=6, nsrc=6,
+                jones=False, nw=4, sups=(7, 7, 9, 9), overs=11, flag_frac=0.0, bda="pervis"):
+    """A small faceted major cycle: nfx x nfx facets on an m x m image (padded facets n px),
+    point sources at pixel centres, visibilities from the FP64 DFT.  Returns (Problem, geometry)."""
+    from .imaging import FacetGeometry
+    cell = cell_arcsec * synth.ARCSEC
+    geom = FacetGeometry(m, nfx, m // nfx, n, cell, mixing_width=4.0)
+    freqs = np.linspace(1.2e9, 1.5e9, nchan)
+    umax = 0.35 / cell
+    diam = umax * C_LIGHT / freqs.max()
+    obs = synth.make_observation("disk", nant, ntime, 900.0, freqs, 45.0, -30.7, seed, diameter_m=diam)
+    rng = np.random.default_rng(seed + 1)
+    X = rng.integers(m // 8, m - m // 8, nsrc)
+    Y = rng.integers(m // 8, m - m // 8, nsrc)
+    flux = rng.uniform(0.5, 2.0, nsrc)
+    l = (X - m // 2) * cell
+    mm = (Y - m // 2) * cell
+    sky = synth.Sky(flux, l, mm)
+    vis = synth.predict_point_sources(obs.uvw, freqs, sky).numpy()
+    R = obs.nrow
+    flags = synth.random_flags(R, nchan, 4, flag_frac, seed + 3)
+    weights = np.ones((R, nchan), np.float32)
+    wmax = float(np.abs(obs.uvw[:, 2]).max()) + 1e-3
+    facets = []
+    for i in range(geom.nf):
+        l0, m0 = geom.lm[i]
+        f = make_cf_facet(l0, m0, n, cell_arcsec, freqs, nw, list(sups), overs, wmax)
+        f.kms_dir = i
+        facets.append(f)
+    if bda == "pervis":
+        bmap = bdamod.bda_map_per_visibility(obs.a0, obs.a1, nchan)
+    else:
+        bmap = bdamod.bda_map_factors(obs.a0, obs.a1, nchan, lambda key: (2, 2))
+    jd = None
+    if jones:
+        jd = dict(mode=1, interp=0, apply_amp=1, apply_phase=1, scale=0, calib_error=0.0,
+                  kms=synth.random_jones(2, geom.nf, nant, 2, seed + 4, 0.05, 0.2),
+                  kms_time=np.minimum((obs.tidx * 2) // ntime, 1).astype(np.int32),
+                  kms_chan=np.minimum((np.arange(nchan) * 2) // nchan, 1).astype(np.int32))
+    chan_map = np.zeros(nchan, np.int32)
+    P = Problem(uvw=obs.uvw, vis=vis, flags=flags, weights=weights, freqs=freqs, a0=obs.a0, a1=obs.a1,
+                chan_map=chan_map, chan_map_degrid=chan_map.copy(), bda_grid=bmap, bda_degrid=bmap,
+                facets=facets, npix=n, nband=1, npol=1, incr=np.array([-cell, cell]),
+                corr_codes=[9, 10, 11, 12], stokes_codes=[1], jones=jd)
+    P.extra.update(sky=sky, src_pix=(X, Y), cell=cell, image_npix=m, name="small_cycle")
+    return P, geom

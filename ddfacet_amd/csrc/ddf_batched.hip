@@ -407,6 +407,7 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
 }
 
 // ---- cycle: grid pass B (LDS tiles) ---------------------------------------------------------------
+template <bool CONJ>
 __global__ __launch_bounds__(256) void grid_pass_b(BatchParams p, const uint64_t* place, const float2* V,
                                                    const BinItem* items, const uint32_t* recs, uint32_t* queue) {
   extern __shared__ float tile[];  // [npg][W][W][2]
@@ -439,7 +440,8 @@ __global__ __launch_bounds__(256) void grid_pass_b(BatchParams p, const uint64_t
         const int ip = c / S2;
         const int kk = c - ip * S2;
         const int sy = kk / S, sx = kk - sy * S;
-        const float2 w = cfp[kk];
+        float2 w = cfp[kk];
+        if (CONJ) w.y = -w.y;
         const float2 val = vv[ip];
         const float re = val.x * w.x - val.y * w.y;
         const float im = val.x * w.y + val.y * w.x;
@@ -620,6 +622,91 @@ __global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v
   }
 }
 
+
+// ---- adjoint of degrid pass B ------------------------------------------------------------------------
+// Forward (degrid_pass_b): vis_k = mask(corr_k * J0 (D M) J1^H) summed over facets.  Adjoint for
+// one (block, facet): Mbar = D^H (sum_k conj(corr_k) J0^H vbar_k J1), mode-2 Jones applied to the sum.
+// Lanes are facets; each lane walks its block's visibilities (all of them: degridding ignores flags).
+template <int NCORR, int JMODE, int PREC>
+__global__ __launch_bounds__(256) void degrid_adj_b(BatchParams p, const VisP v, const uint64_t* place,
+                                                    const float2* vbar, float2* Mbar, uint32_t* queue) {
+  const int G = p.group;
+  const int slots = p.slots;
+  const int slot = threadIdx.x / G;
+  const int lf = threadIdx.x - slot * G;
+  const bool active = slot < slots;
+  __shared__ uint32_t s_item;
+  while (true) {
+    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint32_t item = s_item;
+    if (item >= p.nitems_a) break;
+    const int group = int(int64_t(item) / p.nchunks);
+    const int64_t chunk = int64_t(item) - int64_t(group) * p.nchunks;
+    const int f = group * G + lf;
+    const int64_t b0 = chunk * p.blocks_per_item;
+    const int64_t b1 = min(b0 + p.blocks_per_item, v.nblocks);
+    for (int64_t b = b0 + slot; active && f < p.nf && b < b1; b += slots) {
+      if (!place_valid(place[b * p.nf + f])) continue;
+      const FacetDev& fd = p.facets[f];
+      BlockRec br = read_block(v, b);
+      double fm = 0;
+      for (int c = br.ch0; c < br.ch1; ++c) fm += v.freqs[c];
+      fm /= double(br.ch1 - br.ch0);
+      const int64_t mid = br.rows[br.nr / 2];
+      const bool mid_ok = mid >= 0 && mid < v.nrow;
+      DecorrP dp = p.dp;
+      dp.l0 = fd.dec_l0;
+      dp.m0 = fd.dec_m0;
+      const double dec = mid_ok ? decorr_factor(dp, v.uvw, fm, mid) : 1.;
+      JonesP jp = p.jp;
+      jp.kdir = fd.kms_dir;
+      jp.bdir = fd.beam_dir;
+      cd acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+      for (int r = 0; r < br.nr; ++r) {
+        const int64_t row = br.rows[r];
+        if (row < 0 || row >= v.nrow) continue;
+        const double* uvw = v.uvw + 3 * row;
+        const double angle = 2. * kPi * (uvw[0] * fd.l0 + uvw[1] * fd.m0 + uvw[2] * fd.n0) / kC;
+        for (int c = br.ch0; c < br.ch1; ++c) {
+          const float2* vp = vbar + (size_t(row) * v.nchan + c) * NCORR;
+          M2 m;
+          if (NCORR == 4) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) m.a[q] = cd{vp[q].x, vp[q].y};
+          } else if (NCORR == 2) {
+            m.a[0] = cd{vp[0].x, vp[0].y}; m.a[1] = cd{0, 0}; m.a[2] = cd{0, 0}; m.a[3] = cd{vp[1].x, vp[1].y};
+          } else {
+            m.a[0] = cd{vp[0].x, vp[0].y}; m.a[1] = cd{0, 0}; m.a[2] = cd{0, 0}; m.a[3] = cd{0, 0};
+          }
+          const cd cc = conj(phasor_prec<PREC>(v.freqs[c], angle)) * dec;
+          if (JMODE == 1) {
+            JonesPair jj = jones_at(jp, row, c, uvw, p.wl_mean);
+            m = mul(mul(herm(jj.J0), m), jj.J1);
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[q] = acc[q] + m.a[q] * cc;
+        }
+      }
+      if (JMODE == 2 && mid_ok) {
+        JonesPair jj = jones_at(jp, mid, (br.ch0 + br.ch1) / 2, v.uvw + 3 * mid, p.wl_mean);
+        M2 m{{acc[0], acc[1], acc[2], acc[3]}};
+        m = mul(mul(herm(jj.J0), m), jj.J1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = m.a[q];
+      }
+      float2* out = Mbar + (size_t(b) * p.nf + f) * p.nstokes;
+      for (int s = 0; s < p.nstokes; ++s) {
+        cd t = cd{0, 0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) t = t + conj(cd{p.sm[q][s][0], p.sm[q][s][1]}) * acc[q];
+        out[s] = make_float2(float(t.re), float(t.im));
+      }
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 // ---- host side -----------------------------------------------------------------------------------
@@ -711,7 +798,18 @@ int batched_grid_b(const BatchParams& p, const uint64_t* place, const float2* V,
   const int W = p.tile + 2 * p.halo;
   const int npg = p.nstokes < p.npol ? p.nstokes : p.npol;
   size_t lds = size_t(npg) * W * W * 2 * sizeof(float);
-  hipLaunchKernelGGL(grid_pass_b, dim3(nwg), dim3(256), lds, st, p, place, V, items, recs, queue);
+  hipLaunchKernelGGL(grid_pass_b<false>, dim3(nwg), dim3(256), lds, st, p, place, V, items, recs, queue);
+  return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
+}
+
+// Adjoint of degrid pass A: conj(CF) stamps of M-bar, scattered through the degrid bins.
+int batched_degrid_a_adjoint(const BatchParams& p, const uint64_t* place, const float2* Mbar, const BinItem* items,
+                             const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st) {
+  const int W = p.tile + 2 * p.halo;
+  size_t lds = size_t(p.nstokes) * W * W * 2 * sizeof(float);
+  BatchParams q = p;
+  q.npol = p.npol;
+  hipLaunchKernelGGL(grid_pass_b<true>, dim3(nwg), dim3(256), lds, st, q, place, Mbar, items, recs, queue);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
@@ -747,6 +845,37 @@ int batched_degrid_b(const BatchParams& p, const VisP& v, const float2* M, const
   if (v.ncorr == 4) launch_db_j<4>(p, v, M, vrc, vblk, nvl, unsigned(nb), st);
   else if (v.ncorr == 2) launch_db_j<2>(p, v, M, vrc, vblk, nvl, unsigned(nb), st);
   else launch_db_j<1>(p, v, M, vrc, vblk, nvl, unsigned(nb), st);
+  return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
+}
+
+}  // namespace ddf
+
+namespace ddf {
+
+template <int NCORR, int JMODE>
+static void launch_adj_b_p(const BatchParams& p, const VisP& v, const uint64_t* place, const float2* vbar,
+                           float2* Mbar, uint32_t* q, unsigned nwg, hipStream_t st) {
+  if (p.prec == 1)
+    hipLaunchKernelGGL((degrid_adj_b<NCORR, JMODE, 1>), dim3(nwg), dim3(256), 0, st, p, v, place, vbar, Mbar, q);
+  else if (p.prec == 2)
+    hipLaunchKernelGGL((degrid_adj_b<NCORR, JMODE, 2>), dim3(nwg), dim3(256), 0, st, p, v, place, vbar, Mbar, q);
+  else
+    hipLaunchKernelGGL((degrid_adj_b<NCORR, JMODE, 0>), dim3(nwg), dim3(256), 0, st, p, v, place, vbar, Mbar, q);
+}
+
+template <int NCORR>
+static void launch_adj_b_j(const BatchParams& p, const VisP& v, const uint64_t* place, const float2* vbar,
+                           float2* Mbar, uint32_t* q, unsigned nwg, hipStream_t st) {
+  if (p.jmode == 1) launch_adj_b_p<NCORR, 1>(p, v, place, vbar, Mbar, q, nwg, st);
+  else if (p.jmode == 2) launch_adj_b_p<NCORR, 2>(p, v, place, vbar, Mbar, q, nwg, st);
+  else launch_adj_b_p<NCORR, 0>(p, v, place, vbar, Mbar, q, nwg, st);
+}
+
+int batched_degrid_b_adjoint(const BatchParams& p, const VisP& v, const uint64_t* place, const float2* vbar,
+                             float2* Mbar, uint32_t* queue, unsigned nwg, hipStream_t st) {
+  if (v.ncorr == 4) launch_adj_b_j<4>(p, v, place, vbar, Mbar, queue, nwg, st);
+  else if (v.ncorr == 2) launch_adj_b_j<2>(p, v, place, vbar, Mbar, queue, nwg, st);
+  else launch_adj_b_j<1>(p, v, place, vbar, Mbar, queue, nwg, st);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 

@@ -243,6 +243,31 @@ __global__ __launch_bounds__(256) void model_nonzero_kernel(const ImgFacet* F, i
   }
 }
 
+// Adjoint of the model cut: image_bar[ch, pol, X, Y] = sum over facets (in order) of
+// factor / FacetNorm[X, Y] * SW[a, b] / Sphe[a, b] * Re(plane_bar[ch, pol, n-1-b, a]),
+// zero where Sphe < 1e-3 (the forward's flips and box read backwards).
+__global__ __launch_bounds__(256) void model_cut_adjoint_kernel(const ImgFacet* F, int nf, int n, int nband, int npol,
+                                                                int m, const float* norm, float factor, float* image_bar) {
+  const int64_t total = int64_t(nband) * npol * m * m;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t plane = i / (int64_t(m) * m);
+    const int64_t pix = i - plane * int64_t(m) * m;
+    const int X = int(pix / m), Y = int(pix - int64_t(X) * m);
+    float acc = 0.f;
+    for (int f = 0; f < nf; ++f) {
+      const ImgFacet& fa = F[f];
+      const Edges& e = fa.e;
+      if (X < e.x0m || X >= e.x1m || Y < e.y0m || Y >= e.y1m) continue;
+      const int a = X - e.x0m + e.x0f, b = Y - e.y0m + e.y0f;
+      const size_t ab = size_t(a) * n + b;
+      if (fa.sphe[ab] < 1e-3f) continue;
+      const float g = fa.model[(size_t(plane) * n + (n - 1 - b)) * n + a].x;
+      acc = acc + g * factor / norm[pix] * fa.sw[ab] / fa.sphe[ab];
+    }
+    image_bar[i] = acc;
+  }
+}
+
 __global__ void any_nonzero_kernel(const float* x, int64_t n, int* flag) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     if (x[i] != 0.f) flag[0] = 1;
@@ -400,5 +425,42 @@ extern "C" int ddf_model_to_grid(const ddf_image_facet* facets, int32_t nf, int3
     }
   }
   if (contiguous) return ddf_fft2c(facets[0].model, n, int64_t(nf) * nband * npol, 0, 1.0, stream);
+  return DDF_OK;
+}
+
+extern "C" int ddf_model_to_grid_adjoint(const ddf_image_facet* facets, int32_t nf, int32_t npix_facet, int32_t nband,
+                                         int32_t npol, int32_t m, float* image_bar, const float* facet_norm,
+                                         int32_t overs, void* stream) {
+  if (!facets || nf <= 0 || npix_facet <= 0 || m <= 0 || !image_bar || !facet_norm || nband <= 0 || npol <= 0)
+    return ifail(DDF_EINVAL, "bad arguments");
+  for (int i = 0; i < nf; ++i)
+    if (!facets[i].model) return ifail(DDF_EINVAL, "facet model pointer is null");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int n = npix_facet;
+  // adjoint of the forward centred FFT (1/n^2 fft): the centred inverse FFT scaled by 1/n^2
+  const int64_t per = int64_t(nband) * npol * n * n;
+  bool contiguous = true;
+  for (int f = 1; f < nf; ++f)
+    contiguous &= reinterpret_cast<float2*>(facets[f].model) == reinterpret_cast<float2*>(facets[0].model) + f * per;
+  const double inv = 1.0 / (double(n) * double(n));
+  if (contiguous) {
+    int rc = ddf_fft2c(facets[0].model, n, int64_t(nf) * nband * npol, 1, inv, stream);
+    if (rc) return rc;
+  } else {
+    for (int f = 0; f < nf; ++f) {
+      int rc = ddf_fft2c(facets[f].model, n, int64_t(nband) * npol, 1, inv, stream);
+      if (rc) return rc;
+    }
+  }
+  std::vector<void*> tmp;
+  TmpFree guard{tmp, st};
+  ImgFacet* d = nullptr;
+  int rc = upload_facets(facets, nf, npix_facet, m, st, &d, tmp);
+  if (rc) return rc;
+  const float factor = float(double(overs) * n) * float(double(overs) * n);
+  const int64_t total = int64_t(nband) * npol * m * m;
+  hipLaunchKernelGGL(model_cut_adjoint_kernel, dim3(blocks_for(total)), dim3(256), 0, st, d, nf, n, nband, npol, m,
+                     facet_norm, factor, image_bar);
+  HIP_OK(hipGetLastError());
   return DDF_OK;
 }

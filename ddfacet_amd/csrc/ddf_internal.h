@@ -133,4 +133,9 @@ int batched_degrid_a(const BatchParams& p, const uint64_t* place, float2* M, con
 int batched_degrid_b(const BatchParams& p, const VisP& v, const float2* M, const uint64_t* vrc, const uint32_t* vblk,
                      int64_t nvl, hipStream_t st);
 
+int batched_degrid_b_adjoint(const BatchParams& p, const VisP& v, const uint64_t* place, const float2* vbar,
+                             float2* Mbar, uint32_t* queue, unsigned nwg, hipStream_t st);
+int batched_degrid_a_adjoint(const BatchParams& p, const uint64_t* place, const float2* Mbar, const BinItem* items,
+                             const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st);
+
 }  // namespace ddf

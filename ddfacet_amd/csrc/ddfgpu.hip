@@ -1223,6 +1223,10 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   P->pg.nchunks = (vgrid->nblocks + bpi - 1) / bpi;
   if (int64_t(P->pg.ngroups) * P->pg.nchunks >= (int64_t(1) << 32)) return bail(fail(DDF_EINVAL, "too many work items"));
   P->pg.nitems_a = uint32_t(int64_t(P->pg.ngroups) * P->pg.nchunks);
+  P->pd.blocks_per_item = bpi;
+  P->pd.nchunks = (vdegrid->nblocks + bpi - 1) / bpi;
+  if (int64_t(P->pd.ngroups) * P->pd.nchunks >= (int64_t(1) << 32)) return bail(fail(DDF_EINVAL, "too many work items"));
+  P->pd.nitems_a = uint32_t(int64_t(P->pd.ngroups) * P->pd.nchunks);
   const size_t per_f = size_t(g->nband) * g->npol + 2 * size_t(g->nband);
   if (size_t(nfacets) * per_f * 8 > 40 * 1024) return bail(fail(DDF_EINVAL, "facets x bands too large for one plan"));
   P->pg.chan_lds = (P->pg.jmode != 0) &&
@@ -1296,6 +1300,18 @@ extern "C" int ddf_plan_degrid(ddf_plan* P, float* vis, void* stream) {
     return fail(DDF_EHIP, "degrid pass A launch failed");
   if (P->nvl && batched_degrid_b(P->pd, v, P->M, P->vrc, P->vblk, P->nvl, st))
     return fail(DDF_EHIP, "degrid pass B launch failed");
+  return DDF_OK;
+}
+
+extern "C" int ddf_plan_degrid_adjoint(ddf_plan* P, const float* vis_bar, void* stream) {
+  if (!P || !vis_bar) return fail(DDF_EINVAL, "null argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (batched_reset(P->queues + 2, 2, st)) return fail(DDF_EHIP, "launch failed");
+  const float2* vb = reinterpret_cast<const float2*>(vis_bar);
+  if (P->pd.nitems_a && batched_degrid_b_adjoint(P->pd, P->vd, P->place_d, vb, P->M, P->queues + 2, P->nwg, st))
+    return fail(DDF_EHIP, "degrid adjoint pass B launch failed");
+  if (P->pd.nitems_b && batched_degrid_a_adjoint(P->pd, P->place_d, P->M, P->items_d, P->recs_d, P->queues + 3, P->nwg, st))
+    return fail(DDF_EHIP, "degrid adjoint pass A launch failed");
   return DDF_OK;
 }
 

@@ -191,6 +191,12 @@ int ddf_plan_grid(ddf_plan* plan, float* vis, int32_t dopsf, void* stream);
 /* vis (NULL: vdegrid->vis) -= sum over facets of the degridded models, like one
  * pyDeGridderWPol call per facet.                                                                */
 int ddf_plan_degrid(ddf_plan* plan, float* vis, void* stream);
+/* Adjoint of ddf_plan_degrid as a linear map from the facet model grids to the visibilities:
+ * facets[i].grid += D_i^H vis_bar for every facet (conjugate phasor, J0^H V J1 Jones, transposed
+ * Stokes map, conjugate CF stamps through the degrid map's placements).  No reference
+ * counterpart: it is the backward of the differentiable degridding operator.  The caller zeroes
+ * the grids first; vis_bar is complex64 [nrow][nchan][ncorr].                                   */
+int ddf_plan_degrid_adjoint(ddf_plan* plan, const float* vis_bar, void* stream);
 /* Counters: [0] grid records, [1] degrid records, [2] grid work items, [3] degrid work items,
  * [4] degrid visibilities, [5] device bytes held, [6] tile, [7] facet group.                    */
 int ddf_plan_info(const ddf_plan* plan, int64_t* out, int32_t n);
@@ -227,6 +233,13 @@ int ddf_stitch(const ddf_image_facet* facets, int32_t nf, int32_t npix_facet, in
 int ddf_model_to_grid(const ddf_image_facet* facets, int32_t nf, int32_t npix_facet, int32_t nband,
                       int32_t npol, int32_t m, const float* image, const float* facet_norm,
                       int32_t overs, void* stream);
+
+/* Adjoint of ddf_model_to_grid (no reference counterpart; backward of the differentiable
+ * forward model): takes facets[i].model as the model-grid gradient (overwritten by the adjoint
+ * FFT) and writes image_bar float32 [nband][npol][m][m] (overwritten).                           */
+int ddf_model_to_grid_adjoint(const ddf_image_facet* facets, int32_t nf, int32_t npix_facet, int32_t nband,
+                              int32_t npol, int32_t m, float* image_bar, const float* facet_norm,
+                              int32_t overs, void* stream);
 
 #ifdef __cplusplus
 }
