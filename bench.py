@@ -128,7 +128,20 @@ def main():
     ap.add_argument("--mode", default="plain", choices=["plain", "batched"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--facets", type=int, default=0, help="use only the first N facets (debug)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a device copy of 1 GiB (reads 1 GiB, writes 1 GiB), to scale the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiler run")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        x = torch.ones(2 ** 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4, "launches": 3}))
+        return
     if a.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md for how "
               "the path would shard); run with --gpus 1", file=sys.stderr)
