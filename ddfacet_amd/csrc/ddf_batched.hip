@@ -147,11 +147,13 @@ __global__ __launch_bounds__(256) void plan_place(BatchParams p, const VisP v, u
 }
 
 // ---- plan: binning (counting sort) --------------------------------------------------------------
+// Bin = (facet, band, tile y, tile x, CF plane): the records of one tile are contiguous and, inside
+// the tile, grouped by CF plane so consecutive stamps reuse the same plane (cache locality).
 __device__ __forceinline__ int64_t bin_of(const BatchParams& p, int64_t t, uint64_t rec) {
   const int f = int(t % p.nf);
   Unpacked u = unpack_place(rec);
   const int tx = u.locx / p.tile, ty = u.locy / p.tile;
-  return ((int64_t(f) * p.nband + u.band) * p.ntiles_y + ty) * p.ntiles_x + tx;
+  return (((int64_t(f) * p.nband + u.band) * p.ntiles_y + ty) * p.ntiles_x + tx) * (2 * p.nw) + u.plane;
 }
 
 __global__ __launch_bounds__(256) void plan_count(BatchParams p, int64_t nblocks, const uint64_t* place,
@@ -172,6 +174,28 @@ __global__ __launch_bounds__(256) void plan_fill(BatchParams p, int64_t nblocks,
     int64_t bin = bin_of(p, t, rec);
     uint32_t slot = atomicAdd(&fill[bin], 1u);
     recs[bin_start[bin] + slot] = uint32_t(t);
+  }
+}
+
+__global__ __launch_bounds__(256) void plan_gather_place(const uint64_t* place, const uint32_t* recs, int64_t n,
+                                                         uint64_t* bplace) {
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x)
+    bplace[k] = place[recs[k]];
+}
+
+// Jones table [t][dir][ant][chan][4] -> [t][ant][chan][dir][4]
+__global__ __launch_bounds__(256) void plan_transpose_jones(const float2* src, float2* dst, int nt, int nd, int na,
+                                                            int nch) {
+  const int64_t total = int64_t(nt) * nd * na * nch;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = i;
+    const int ch = int(r % nch); r /= nch;
+    const int a = int(r % na); r /= na;
+    const int d = int(r % nd); r /= nd;
+    const int t = int(r);
+    const int64_t o = ((int64_t(t) * na + a) * nch + ch) * nd + d;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[o * 4 + q] = src[i * 4 + q];
   }
 }
 
@@ -302,6 +326,8 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
         BBblk = jblk.BB;
       }
       double* sch = s_chan + lf * 2 * v.nchan;
+      JonesCache jc;
+      jc.reset();
       for (int r = 0; r < br.nr; ++r) {
         const int64_t row = br.rows[r];
         if (row < 0 || row >= v.nrow) continue;
@@ -329,7 +355,7 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
           cd wt = cd{fw, 0};
           if (!p.dopsf) wt = phasor_prec<PREC>(v.freqs[c], angle) * fw;
           if (JMODE == 1) {
-            JonesPair jj = jones_at(jp, row, c, uvw, p.wl_mean);
+            const JonesPair& jj = jc.get(jp, row, c, uvw, p.wl_mean);
             M2 m;
             if (p.dopsf) {
               m = mul(mul(herm(jj.J0), mul(jj.J0, herm(jj.J1))), jj.J1);
@@ -408,7 +434,7 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
 
 // ---- cycle: grid pass B (LDS tiles) ---------------------------------------------------------------
 template <bool CONJ>
-__global__ __launch_bounds__(256) void grid_pass_b(BatchParams p, const uint64_t* place, const float2* V,
+__global__ __launch_bounds__(256) void grid_pass_b(BatchParams p, const uint64_t* bplace, const float2* V,
                                                    const BinItem* items, const uint32_t* recs, uint32_t* queue) {
   extern __shared__ float tile[];  // [npg][W][W][2]
   const int W = p.tile + 2 * p.halo;
@@ -428,27 +454,40 @@ __global__ __launch_bounds__(256) void grid_pass_b(BatchParams p, const uint64_t
     const int f = bi.facet;
     const FacetDev& fd = p.facets[f];
     const int x0 = bi.tx * p.tile - p.halo, y0 = bi.ty * p.tile - p.halo;
-    for (uint32_t k = bi.begin + wave; k < bi.end; k += nwave) {
-      const uint32_t t = recs[k];
-      const uint64_t rec = place[t];
-      const Unpacked u = unpack_place(rec);
-      const int S = plane_support(p, u.plane);
-      const int S2 = S * S, sup = (S - 1) / 2;
-      const float2* cfp = fd.cf + fd.cf_offset[u.plane] + int64_t(u.io * p.overs + u.jo) * S2;
-      const float2* vv = V + size_t(t) * p.nstokes;
-      for (int c = lane; c < S2 * npg; c += 64) {
-        const int ip = c / S2;
-        const int kk = c - ip * S2;
-        const int sy = kk / S, sx = kk - sy * S;
-        float2 w = cfp[kk];
-        if (CONJ) w.y = -w.y;
-        const float2 val = vv[ip];
-        const float re = val.x * w.x - val.y * w.y;
-        const float im = val.x * w.y + val.y * w.x;
-        const int ty = u.locy - sup + sy - y0, tx = u.locx - sup + sx - x0;
-        float* cell = tile + ((size_t(ip) * W + ty) * W + tx) * 2;
-        atomicAdd(cell, re);
-        atomicAdd(cell + 1, im);
+    // each wavefront takes 64 records at a time: lane j fetches record j's placement and values,
+    // then the wavefront stamps them one after the other (broadcast from lane j)
+    for (uint32_t k0 = bi.begin + 64u * wave; k0 < bi.end; k0 += 64u * nwave) {
+      const uint32_t k = k0 + lane;
+      const bool ok = k < bi.end;
+      const uint64_t my_rec = ok ? bplace[k] : 0;
+      const uint32_t my_t = ok ? recs[k] : 0;
+      float2 my_v[4];
+#pragma unroll
+      for (int ip = 0; ip < 4; ++ip) my_v[ip] = (ok && ip < npg) ? V[size_t(my_t) * p.nstokes + ip] : make_float2(0.f, 0.f);
+      const int cnt = int(min(64u, bi.end - k0));
+      for (int j = 0; j < cnt; ++j) {
+        const uint64_t rec = __shfl(my_rec, j, 64);
+        const Unpacked u = unpack_place(rec);
+        const int S = plane_support(p, u.plane);
+        const int S2 = S * S, sup = (S - 1) / 2;
+        const float2* cfp = fd.cf + fd.cf_offset[u.plane] + int64_t(u.io * p.overs + u.jo) * S2;
+        float2 vv[4];
+#pragma unroll
+        for (int ip = 0; ip < 4; ++ip) vv[ip] = make_float2(__shfl(my_v[ip].x, j, 64), __shfl(my_v[ip].y, j, 64));
+        for (int c = lane; c < S2 * npg; c += 64) {
+          const int ip = c / S2;
+          const int kk = c - ip * S2;
+          const int sy = kk / S, sx = kk - sy * S;
+          float2 w = cfp[kk];
+          if (CONJ) w.y = -w.y;
+          const float2 val = ip == 0 ? vv[0] : (ip == 1 ? vv[1] : (ip == 2 ? vv[2] : vv[3]));
+          const float re = val.x * w.x - val.y * w.y;
+          const float im = val.x * w.y + val.y * w.x;
+          const int ty = u.locy - sup + sy - y0, tx = u.locx - sup + sx - x0;
+          float* cell = tile + ((size_t(ip) * W + ty) * W + tx) * 2;
+          atomicAdd(cell, re);
+          atomicAdd(cell + 1, im);
+        }
       }
     }
     __syncthreads();
@@ -469,7 +508,7 @@ __global__ __launch_bounds__(256) void grid_pass_b(BatchParams p, const uint64_t
 }
 
 // ---- cycle: degrid pass A (LDS-staged gather) --------------------------------------------------
-__global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64_t* place, float2* M,
+__global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64_t* bplace, float2* M,
                                                      const BinItem* items, const uint32_t* recs, uint32_t* queue) {
   extern __shared__ float tile[];  // [nstokes][W][W][2]
   const int W = p.tile + 2 * p.halo;
@@ -501,31 +540,46 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
       tile[2 * i + 1] = im;
     }
     __syncthreads();
-    for (uint32_t k = bi.begin + wave; k < bi.end; k += nwave) {
-      const uint32_t t = recs[k];
-      const Unpacked u = unpack_place(place[t]);
-      const int S = plane_support(p, u.plane);
-      const int S2 = S * S, sup = (S - 1) / 2;
-      const float2* cfp = fd.cf + fd.cf_offset[u.plane] + int64_t(u.io * p.overs + u.jo) * S2;
-      double sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
-      for (int c = lane; c < S2 * p.nstokes; c += 64) {
-        const int ip = c / S2;
-        const int kk = c - ip * S2;
-        const int sy = kk / S, sx = kk - sy * S;
-        const int ty = u.locy - sup + sy - y0, tx = u.locx - sup + sx - x0;
-        const float* cell = tile + ((size_t(ip) * W + ty) * W + tx) * 2;
-        const float2 w = cfp[kk];
-        const float pr = cell[0] * w.x - cell[1] * w.y;
-        const float pi = cell[0] * w.y + cell[1] * w.x;
-        if (ip == 0) { sr[0] += pr; si[0] += pi; }
-        else if (ip == 1) { sr[1] += pr; si[1] += pi; }
-        else if (ip == 2) { sr[2] += pr; si[2] += pi; }
-        else { sr[3] += pr; si[3] += pi; }
+    for (uint32_t k0 = bi.begin + 64u * wave; k0 < bi.end; k0 += 64u * nwave) {
+      const uint32_t k = k0 + lane;
+      const bool ok = k < bi.end;
+      const uint64_t my_rec = ok ? bplace[k] : 0;
+      const uint32_t my_t = ok ? recs[k] : 0;
+      float2 res[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+      const int cnt = int(min(64u, bi.end - k0));
+      for (int j = 0; j < cnt; ++j) {
+        const uint64_t rec = __shfl(my_rec, j, 64);
+        const Unpacked u = unpack_place(rec);
+        const int S = plane_support(p, u.plane);
+        const int S2 = S * S, sup = (S - 1) / 2;
+        const float2* cfp = fd.cf + fd.cf_offset[u.plane] + int64_t(u.io * p.overs + u.jo) * S2;
+        double sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
+        for (int c = lane; c < S2 * p.nstokes; c += 64) {
+          const int ip = c / S2;
+          const int kk = c - ip * S2;
+          const int sy = kk / S, sx = kk - sy * S;
+          const int ty = u.locy - sup + sy - y0, tx = u.locx - sup + sx - x0;
+          const float* cell = tile + ((size_t(ip) * W + ty) * W + tx) * 2;
+          const float2 w = cfp[kk];
+          const float pr = cell[0] * w.x - cell[1] * w.y;
+          const float pi = cell[0] * w.y + cell[1] * w.x;
+          if (ip == 0) { sr[0] += pr; si[0] += pi; }
+          else if (ip == 1) { sr[1] += pr; si[1] += pi; }
+          else if (ip == 2) { sr[2] += pr; si[2] += pi; }
+          else { sr[3] += pr; si[3] += pi; }
+        }
+#pragma unroll
+        for (int ip = 0; ip < 4; ++ip) {
+          if (ip < p.nstokes) {
+            const double a = wave_sum(sr[ip]), b = wave_sum(si[ip]);
+            if (lane == j) res[ip] = make_float2(float(a), float(b));
+          }
+        }
       }
-      float2* out = M + size_t(t) * p.nstokes;
-      for (int ip = 0; ip < p.nstokes; ++ip) {
-        const double a = wave_sum(sr[ip]), b = wave_sum(si[ip]);
-        if (lane == 0) out[ip] = make_float2(float(a), float(b));
+      if (ok) {
+#pragma unroll
+        for (int ip = 0; ip < 4; ++ip)
+          if (ip < p.nstokes) M[size_t(my_t) * p.nstokes + ip] = res[ip];
       }
     }
     __syncthreads();
@@ -876,6 +930,28 @@ int batched_degrid_b_adjoint(const BatchParams& p, const VisP& v, const uint64_t
   if (v.ncorr == 4) launch_adj_b_j<4>(p, v, place, vbar, Mbar, queue, nwg, st);
   else if (v.ncorr == 2) launch_adj_b_j<2>(p, v, place, vbar, Mbar, queue, nwg, st);
   else launch_adj_b_j<1>(p, v, place, vbar, Mbar, queue, nwg, st);
+  return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
+}
+
+}  // namespace ddf
+
+namespace ddf {
+
+int batched_gather_place(const uint64_t* place, const uint32_t* recs, int64_t n, uint64_t* bplace, hipStream_t st) {
+  if (n <= 0) return DDF_OK;
+  int64_t nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(plan_gather_place, dim3(unsigned(nb)), dim3(256), 0, st, place, recs, n, bplace);
+  return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
+}
+
+int batched_transpose_jones(const float2* src, float2* dst, const int* dims, hipStream_t st) {
+  int64_t total = int64_t(dims[0]) * dims[1] * dims[2] * dims[3];
+  int64_t nb = (total + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(plan_transpose_jones, dim3(unsigned(nb)), dim3(256), 0, st, src, dst, dims[0], dims[1], dims[2],
+                     dims[3]);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 

@@ -72,6 +72,7 @@ __device__ __forceinline__ cd phasor(double phase) {
 // ---- Jones tables (JonesServer.cc) -----------------------------------------------------------
 struct JonesP {
   int enabled, mode, interp;
+  int tl;  // table layout: 0 the reference [t][dir][ant][chan][4], 1 facet-fastest [t][ant][chan][dir][4]
   const float2* kms;
   int kdims[4];
   const int32_t *ktime, *kchan;
@@ -93,12 +94,13 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 
 // GiveJones (JonesServer.cc:54-95); indices clamped to the table so a bad map cannot fault.
 __device__ inline M2 give_jones(const float2* tab, const int* dims, int it, int ant, int dir, int ich,
-                                int interp, const float* coefs, int ncoefs) {
+                                int interp, const float* coefs, int ncoefs, int tl) {
   it = clampi(it, 0, dims[0] - 1);
   dir = clampi(dir, 0, dims[1] - 1);
   ant = clampi(ant, 0, dims[2] - 1);
   ich = clampi(ich, 0, dims[3] - 1);
-  size_t off = ((size_t(it) * dims[1] + dir) * size_t(dims[2]) + ant) * size_t(dims[3]) + ich;
+  size_t off = tl ? ((size_t(it) * dims[2] + ant) * size_t(dims[3]) + ich) * size_t(dims[1]) + dir
+                  : ((size_t(it) * dims[1] + dir) * size_t(dims[2]) + ant) * size_t(dims[3]) + ich;
   const float2* p = tab + off * 4;
   M2 out;
   if (interp == 0) {
@@ -165,13 +167,13 @@ __device__ inline JonesPair jones_at(const JonesP& jp, int64_t row, int chan, co
   M2 J0 = unity(), J1 = unity();
   if (jp.beam) {
     int t = jp.btime[row], c = jp.bchan[chan];
-    J0 = give_jones(jp.beam, jp.bdims, t, a0, jp.bdir, c, jp.interp, jp.coefs, jp.ncoefs);
-    J1 = give_jones(jp.beam, jp.bdims, t, a1, jp.bdir, c, jp.interp, jp.coefs, jp.ncoefs);
+    J0 = give_jones(jp.beam, jp.bdims, t, a0, jp.bdir, c, jp.interp, jp.coefs, jp.ncoefs, jp.tl);
+    J1 = give_jones(jp.beam, jp.bdims, t, a1, jp.bdir, c, jp.interp, jp.coefs, jp.ncoefs, jp.tl);
   }
   if (jp.kms) {
     int t = jp.ktime[row], c = jp.kchan[chan];
-    M2 K0 = give_jones(jp.kms, jp.kdims, t, a0, jp.kdir, c, jp.interp, jp.coefs, jp.ncoefs);
-    M2 K1 = give_jones(jp.kms, jp.kdims, t, a1, jp.kdir, c, jp.interp, jp.coefs, jp.ncoefs);
+    M2 K0 = give_jones(jp.kms, jp.kdims, t, a0, jp.kdir, c, jp.interp, jp.coefs, jp.ncoefs, jp.tl);
+    M2 K1 = give_jones(jp.kms, jp.kdims, t, a1, jp.kdir, c, jp.interp, jp.coefs, jp.ncoefs, jp.tl);
     norm_jones(jp, K0, uvw, wl_mean);
     norm_jones(jp, K1, uvw, wl_mean);
     J0 = mul(K0, J0);
@@ -184,6 +186,28 @@ __device__ inline JonesPair jones_at(const JonesP& jp, int64_t row, int chan, co
   r.BB = bb * bb;
   return r;
 }
+
+// Per-lane Jones cache with the refresh rule of JonesServer::updateJones (JonesServer.cc:193-298):
+// the Jones pair is re-evaluated when the baseline, a time index or a Jones channel changes, with
+// the uvw of the visibility that triggered the refresh (which only the ScaleJones term reads).
+// Beam and DD parts are refreshed together (the reference refreshes them separately; the two
+// differ only with ScaleJones on and a beam-only key change).
+struct JonesCache {
+  int a0, a1, kt, kc, bt, bc;
+  JonesPair cur;
+  __device__ __forceinline__ void reset() { a0 = a1 = kt = kc = bt = bc = -1; }
+  __device__ __forceinline__ const JonesPair& get(const JonesP& jp, int64_t row, int chan, const double* uvw,
+                                                   double wl_mean) {
+    const int r0 = jp.a0[row], r1 = jp.a1[row];
+    const int k_t = jp.kms ? jp.ktime[row] : -1, k_c = jp.kms ? jp.kchan[chan] : -1;
+    const int b_t = jp.beam ? jp.btime[row] : -1, b_c = jp.beam ? jp.bchan[chan] : -1;
+    if (r0 != a0 || r1 != a1 || k_t != kt || k_c != kc || b_t != bt || b_c != bc) {
+      cur = jones_at(jp, row, chan, uvw, wl_mean);
+      a0 = r0; a1 = r1; kt = k_t; kc = k_c; bt = b_t; bc = b_c;
+    }
+    return cur;
+  }
+};
 
 // ---- decorrelation (DecorrelationHelper.cc:42-63) -------------------------------------------
 struct DecorrP {

@@ -138,4 +138,7 @@ int batched_degrid_b_adjoint(const BatchParams& p, const VisP& v, const uint64_t
 int batched_degrid_a_adjoint(const BatchParams& p, const uint64_t* place, const float2* Mbar, const BinItem* items,
                              const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st);
 
+int batched_gather_place(const uint64_t* place, const uint32_t* recs, int64_t n, uint64_t* bplace, hipStream_t st);
+int batched_transpose_jones(const float2* src, float2* dst, const int* dims, hipStream_t st);
+
 }  // namespace ddf

@@ -913,6 +913,8 @@ struct ddf_plan {
   uint64_t* place_d = nullptr;
   uint32_t* recs_g = nullptr;
   uint32_t* recs_d = nullptr;
+  uint64_t* bplace_g = nullptr;  // placement records in bin order
+  uint64_t* bplace_d = nullptr;
   BinItem* items_g = nullptr;
   BinItem* items_d = nullptr;
   float2* V = nullptr;
@@ -1012,7 +1014,9 @@ void fill_batch(BatchParams& p, const ddf_grid_desc* g, const ddf_facet* f0, con
 // Count -> scan -> fill of the valid records, then the work items (largest first).
 int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place, int chunk, uint32_t** recs_out,
                BinItem** items_out, uint32_t* nitems, int64_t* nrec, hipStream_t st) {
-  const int64_t nbins = int64_t(p.nf) * p.nband * p.ntiles_y * p.ntiles_x;
+  const int64_t nsub = 2 * int64_t(p.nw);  // CF-plane sub-bins of each tile
+  const int64_t ntile_all = int64_t(p.nf) * p.nband * p.ntiles_y * p.ntiles_x;
+  const int64_t nbins = ntile_all * nsub;
   uint32_t* counts = P->alloc<uint32_t>(size_t(nbins));
   uint32_t* fill = P->alloc<uint32_t>(size_t(nbins));
   int64_t* bstart = P->alloc<int64_t>(size_t(nbins));
@@ -1026,9 +1030,12 @@ int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place
   std::vector<int64_t> hs(static_cast<size_t>(nbins));
   int64_t tot = 0;
   std::vector<BinItem> items;
-  for (int64_t i = 0; i < nbins; ++i) {
-    hs[size_t(i)] = tot;
-    const int64_t c = hc[size_t(i)];
+  for (int64_t i = 0; i < ntile_all; ++i) {
+    int64_t c = 0;
+    for (int64_t q = 0; q < nsub; ++q) {
+      hs[size_t(i * nsub + q)] = tot + c;
+      c += hc[size_t(i * nsub + q)];
+    }
     if (c) {
       int64_t rem = i;
       const int tx = int(rem % p.ntiles_x); rem /= p.ntiles_x;
@@ -1049,8 +1056,8 @@ int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place
     tot += c;
   }
   if (tot >= (int64_t(1) << 32)) return fail(DDF_EINVAL, "too many (block, facet) records for one plan");
-  std::stable_sort(items.begin(), items.end(),
-                   [](const BinItem& a, const BinItem& b) { return (a.end - a.begin) > (b.end - b.begin); });
+  // items stay in facet-major order: workgroups that run at the same time work on few facets, so
+  // the CF planes they read stay in cache
   HIP_TRY(hipMemcpyAsync(bstart, hs.data(), size_t(nbins) * 8, hipMemcpyHostToDevice, st));
   uint32_t* recs = P->alloc<uint32_t>(size_t(tot));
   BinItem* d_items = P->alloc<BinItem>(items.size());
@@ -1128,6 +1135,24 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   fill_batch(P->pg, g, &facets[0], smg, fm, wl);
   fill_batch(P->pd, g, &facets[0], smd, fm, wl);
   P->pg.dopsf = g->dopsf;
+  // facet-fastest copies of the Jones tables: the lanes of a wavefront are facets, so their
+  // Jones reads become contiguous
+  for (int k = 0; k < 2; ++k) {
+    const ddf_jones& j = facets[0].jones;
+    const float* src = k ? j.beam : j.kms;
+    const int32_t* dims = k ? j.beam_dims : j.kms_dims;
+    if (!j.enabled || !src) continue;
+    const size_t n = size_t(dims[0]) * dims[1] * dims[2] * dims[3] * 4;
+    float2* dst = P->alloc<float2>(n);
+    if (!dst) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
+    if (batched_transpose_jones(reinterpret_cast<const float2*>(src), dst, dims, st))
+      return bail(fail(DDF_EHIP, "Jones transpose failed"));
+    for (BatchParams* bp : {&P->pg, &P->pd}) {
+      if (k) bp->jp.beam = dst;
+      else bp->jp.kms = dst;
+      bp->jp.tl = 1;
+    }
+  }
   ddf_plan_opts o{};
   if (opts) o = *opts;
   const int tile = o.tile > 0 ? o.tile : 32;
@@ -1212,11 +1237,17 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   P->pg.nitems_b = ni;
   P->stats[0] = nr;
   P->stats[2] = ni;
+  P->bplace_g = P->alloc<uint64_t>(size_t(nr));
+  if (!P->bplace_g || batched_gather_place(P->place_g, P->recs_g, nr, P->bplace_g, st))
+    return bail(fail(DDF_ENOMEM, "plan allocation failed"));
   rc = build_bins(P, P->pd, P->vd, P->place_d, chunk, &P->recs_d, &P->items_d, &ni, &nr, st);
   if (rc) return bail(rc);
   P->pd.nitems_b = ni;
   P->stats[1] = nr;
   P->stats[3] = ni;
+  P->bplace_d = P->alloc<uint64_t>(size_t(nr));
+  if (!P->bplace_d || batched_gather_place(P->place_d, P->recs_d, nr, P->bplace_d, st))
+    return bail(fail(DDF_ENOMEM, "plan allocation failed"));
   // pass A layout
   const int64_t bpi = o.blocks_per_item > 0 ? o.blocks_per_item : 128;
   P->pg.blocks_per_item = bpi;
@@ -1285,7 +1316,7 @@ extern "C" int ddf_plan_grid(ddf_plan* P, float* vis, int32_t dopsf, void* strea
   p.dopsf = dopsf;
   if (batched_reset(P->queues, 2, st)) return fail(DDF_EHIP, "launch failed");
   if (p.nitems_a && batched_grid_a(p, v, P->place_g, P->V, P->queues, P->nwg, st)) return fail(DDF_EHIP, "grid pass A launch failed");
-  if (p.nitems_b && batched_grid_b(p, P->place_g, P->V, P->items_g, P->recs_g, P->queues + 1, P->nwg, st))
+  if (p.nitems_b && batched_grid_b(p, P->bplace_g, P->V, P->items_g, P->recs_g, P->queues + 1, P->nwg, st))
     return fail(DDF_EHIP, "grid pass B launch failed");
   return DDF_OK;
 }
@@ -1296,7 +1327,7 @@ extern "C" int ddf_plan_degrid(ddf_plan* P, float* vis, void* stream) {
   VisP v = P->vd;
   if (vis) v.vis = reinterpret_cast<float2*>(vis);
   if (batched_reset(P->queues + 2, 1, st)) return fail(DDF_EHIP, "launch failed");
-  if (P->pd.nitems_b && batched_degrid_a(P->pd, P->place_d, P->M, P->items_d, P->recs_d, P->queues + 2, P->nwg, st))
+  if (P->pd.nitems_b && batched_degrid_a(P->pd, P->bplace_d, P->M, P->items_d, P->recs_d, P->queues + 2, P->nwg, st))
     return fail(DDF_EHIP, "degrid pass A launch failed");
   if (P->nvl && batched_degrid_b(P->pd, v, P->M, P->vrc, P->vblk, P->nvl, st))
     return fail(DDF_EHIP, "degrid pass B launch failed");
@@ -1310,7 +1341,7 @@ extern "C" int ddf_plan_degrid_adjoint(ddf_plan* P, const float* vis_bar, void* 
   const float2* vb = reinterpret_cast<const float2*>(vis_bar);
   if (P->pd.nitems_a && batched_degrid_b_adjoint(P->pd, P->vd, P->place_d, vb, P->M, P->queues + 2, P->nwg, st))
     return fail(DDF_EHIP, "degrid adjoint pass B launch failed");
-  if (P->pd.nitems_b && batched_degrid_a_adjoint(P->pd, P->place_d, P->M, P->items_d, P->recs_d, P->queues + 3, P->nwg, st))
+  if (P->pd.nitems_b && batched_degrid_a_adjoint(P->pd, P->bplace_d, P->M, P->items_d, P->recs_d, P->queues + 3, P->nwg, st))
     return fail(DDF_EHIP, "degrid adjoint pass A launch failed");
   return DDF_OK;
 }
