@@ -1,4 +1,4 @@
-"""Throughput of one grid + degrid major cycle over all facets (vis/s), cfg2 (81-facet MeerKAT-like).
+"""Throughput of one grid + degrid major cycle over all facets (vis/s), 81-facet MeerKAT-like.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode batched|plain] [--no-cpu]
 
@@ -7,13 +7,22 @@ and subtract them from the visibilities (vis -= sum over facets).  Inputs are re
 before timing starts.  value = Nvis / t_step with Nvis = rows x channels (4-correlation
 visibilities presented, flagged ones included).
 
-Roofline: HBM-bound scatter/gather.  Algorithmic bytes per visibility-facet pair (one 4-corr
-visibility gridded to and degridded from one facet): grid 32 (vis) + 4 (flag) + 4 (weight)
-+ 24/C (uvw); degrid 32 + 32 (vis read + write) + 24/C; plus per facet per cycle the grid
-(zero + write), the model grid read, the BDA maps and the CF tables.  achieved = those bytes
-of the dominant kernel / its mean launch time (HIP events on the launch stream).
-The CPU baseline is the reference gridder (oracle/_ref, built from the reference sources) on
-a stride sample of the same BDA blocks and a few facets, scaled to the full cycle.
+Roofline (HBM-bound scatter/gather).  `roofline` is for the dominant pass (the batched pass with
+the largest mean duration, timed live with HIP events on its launch stream through
+ddf_plan_timing): achieved = that pass's algorithmic bytes / its mean duration, with
+  grid pass A    per vis 40 (vis 32, flags 4, weight 4), per row 24 (uvw), per BDA word 4,
+                 per record 8 (placement) + 8 per Stokes (block sum written), Jones tables once
+  grid pass B    per record 12 (placement 8, index 4) + 8 per Stokes (block sum), per facet the CF
+                 table once and the grid read + written once (2 x grid bytes)
+  degrid pass A  per record 12 + 8 per Stokes (written), per facet the CF table and the model grid
+  degrid pass B  per vis 64 (read + write) + 12 (vis list), per row 24, per record 8 per Stokes,
+                 Jones tables once.
+`roofline.cycle` is the cycle-level figure of the per-facet algorithm: per visibility-facet pair
+40 + 64 + 48 / C bytes, plus per facet 3 grids, the BDA maps and the CF table, over t_step; a
+design that reads a visibility once for all facets can exceed 1 there.
+The CPU baseline is the reference gridder (oracle/_ref, built from the reference sources) on the
+host cores (at most 16 worker processes, one facet each) over a stride sample of the same BDA
+blocks, scaled to the full cycle; it runs before the GPU is initialised.
 """
 import argparse
 import json
@@ -33,37 +42,87 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(P, nf, kind):
+def _cf_bytes(P):
+    f = P.facets[0]
+    return sum(p.size * 8 for p in f.wplanes) + sum(p.size * 8 for p in f.wplanes_conj)
+
+
+def _jones_bytes(P):
+    if P.jones is None:
+        return 0
+    return int(np.asarray(P.jones["kms"]).size) * 8
+
+
+def cycle_bytes(P, nf):
+    """Bytes of one major cycle of the per-facet algorithm (each facet a pass over the
+    visibilities): per visibility-facet pair 40 (grid in) + 64 (degrid read + write) + 48 / C
+    (uvw in both passes); per facet 3 grids (zero/write after gridding, model read), the BDA maps
+    and the CF table."""
     R, C = P.uvw.shape[0], P.freqs.shape[0]
     nvis = R * C
     gbytes = P.nband * P.npol * P.npix * P.npix * 8
-    bda = 4 * (P.bda_grid.size if kind == "grid" else P.bda_degrid.size)
-    cfb = 0
-    for f in P.facets[:nf]:
-        cfb += sum(p.size * 8 for p in f.wplanes) + sum(p.size * 8 for p in f.wplanes_conj)
-    cfb //= max(1, len(P.facets[:nf]))
+    per_facet = 3 * gbytes + 4 * (P.bda_grid.size + P.bda_degrid.size) + _cf_bytes(P)
+    return nf * (nvis * (104 + 48.0 / C) + per_facet)
+
+
+def pass_bytes(P, nf, info, ns):
+    """Algorithmic bytes of one launch of each batched pass (see the module docstring)."""
+    R, C = P.uvw.shape[0], P.freqs.shape[0]
+    nvis = R * C
+    gbytes = P.nband * P.npol * P.npix * P.npix * 8
+    ng, nd = info["grid_records"], info["degrid_records"]
+    cf = _cf_bytes(P)
+    jb = _jones_bytes(P)
+    return {
+        "grid_pass_a": nvis * 40 + R * 24 + 4 * P.bda_grid.size + ng * (8 + 8 * ns) + jb,
+        "grid_pass_b": ng * (12 + 8 * ns) + nf * (cf + 2 * gbytes),
+        "degrid_pass_a": nd * (12 + 8 * ns) + nf * (cf + gbytes),
+        "degrid_pass_b": info["degrid_vis"] * 76 + R * 24 + nd * 8 * ns + jb,
+    }
+
+
+def plain_bytes(P, kind):
+    """Algorithmic bytes of one per-facet plain launch (grid_plain / degrid_plain)."""
+    R, C = P.uvw.shape[0], P.freqs.shape[0]
+    nvis = R * C
+    gbytes = P.nband * P.npol * P.npix * P.npix * 8
     if kind == "grid":
-        per_vis = 40 + 24.0 / C
-        per_facet = 2 * gbytes + bda + cfb  # zero + accumulate
-    else:
-        per_vis = 64 + 24.0 / C
-        per_facet = gbytes + bda + cfb
-    return nf * (nvis * per_vis + per_facet)
+        return nvis * (40 + 24.0 / C) + 2 * gbytes + 4 * P.bda_grid.size + _cf_bytes(P)
+    return nvis * (64 + 24.0 / C) + gbytes + 4 * P.bda_degrid.size + _cf_bytes(P)
 
 
-def cpu_baseline(P, target_s=12.0):
-    """Reference gridder on a stride sample of the BDA blocks, `nfs` facets, one core."""
+_CPU = {}
+
+
+def _cpu_facet_job(fi):
+    """Grid then degrid the sample onto facet `fi` with the reference module (forked worker)."""
+    mod, S = _CPU["mod"], _CPU["S"]
+    g = S.new_grid()
+    sw = np.zeros((S.nband, S.npol))
+    mod.pyGridderWPol32(*S.grid_args(fi, g, sw))
+    v = S.vis.copy()
+    mod.pyDeGridderWPol(*S.degrid_args(fi, g, v))
+    return fi
+
+
+def cpu_baseline(P, target_s=8.0, max_workers=16):
+    """The reference gridder (oracle/_ref, built from DDFacet/Gridder) on the host cores, before
+    the GPU is touched: one facet per worker process (the reference's process pool runs one facet
+    job per core with one OpenMP thread, AsyncProcessPool.py:831-861), each worker gridding then
+    degridding a stride sample of the BDA blocks; 3373 row semaphores as the reference deploys.
+    value = vis-facet pairs per second over all workers / the number of facets."""
+    import multiprocessing as mp
     from ddfacet_amd import bda as bdamod
     from ddfacet_amd.problems import Problem
     from oracle import refmod
     if not refmod.available():
         return None
-    mod = refmod.module()
-    vis_full = P.vis
-    import torch
+    mod = refmod.module(nsem=3373)
+    cores = max(1, min(max_workers, len(os.sched_getaffinity(0))))
     nb = bdamod.nblocks_of(P.bda_grid)
     starts = bdamod.block_starts(P.bda_grid)
     sizes = P.bda_grid[2:2 + nb]
+    rng = np.random.default_rng(5)
 
     def sample(stride):
         sel = np.arange(0, nb, stride)
@@ -78,15 +137,14 @@ def cpu_baseline(P, target_s=12.0):
             szs.append(sizes[b])
         m = np.concatenate([np.array([sel.size, 0], np.int64), np.array(szs, np.int64)] + recs).astype(np.int32)
         nvis_s = int(sum((r[1] - r[0]) * (r.size - 2) for r in recs))
-        if isinstance(vis_full, torch.Tensor):
-            vs = vis_full[torch.as_tensor(rows, device=vis_full.device)].cpu().numpy()
-        else:
-            vs = vis_full[rows]
+        # values do not change the work: unit-variance noise in place of the sky
+        shape = (rows.size, P.freqs.size, 4)
+        vs = (rng.standard_normal(shape, dtype=np.float32) + 1j * rng.standard_normal(shape, dtype=np.float32))
         j = None
         if P.jones is not None:
             j = dict(P.jones)
             j["kms_time"] = P.jones["kms_time"][rows]
-        S = Problem(uvw=np.ascontiguousarray(P.uvw[rows]), vis=np.ascontiguousarray(vs),
+        S = Problem(uvw=np.ascontiguousarray(P.uvw[rows]), vis=np.ascontiguousarray(vs.astype(np.complex64)),
                     flags=np.ascontiguousarray(P.flags[rows]), weights=np.ascontiguousarray(P.weights[rows]),
                     freqs=P.freqs, a0=np.int32(P.a0[rows]), a1=np.int32(P.a1[rows]), chan_map=P.chan_map,
                     chan_map_degrid=P.chan_map_degrid, bda_grid=m, bda_degrid=m, facets=P.facets,
@@ -94,30 +152,167 @@ def cpu_baseline(P, target_s=12.0):
                     stokes_codes=P.stokes_codes, jones=j)
         return S, nvis_s
 
-    def run(S, facets):
-        t0 = time.perf_counter()
-        for fi in facets:
-            g = S.new_grid()
-            s = np.zeros((S.nband, S.npol))
-            mod.pyGridderWPol32(*S.grid_args(fi, g, s))
-            v = S.vis.copy()
-            mod.pyDeGridderWPol(*S.degrid_args(fi, g, v))
-        return time.perf_counter() - t0
-
-    S, nv = sample(400)
     nf = len(P.facets)
-    facets = [0, nf // 2, nf - 1]
-    t = run(S, facets)
-    scale = max(1, int(target_s / max(t, 1e-3)))
-    stride = max(1, 400 // scale)
+    # calibrate the stride on one facet in this process, then one facet per core in parallel
+    _CPU.update(mod=mod)
+    S, nv = sample(400)
+    _CPU["S"] = S
+    t0 = time.perf_counter()
+    _cpu_facet_job(nf // 2)
+    t1 = time.perf_counter() - t0
+    stride = int(max(1, min(400, round(400 * t1 / target_s))))
     S, nv = sample(stride)
-    t = run(S, facets)
-    # vis-facet pairs per second on one core, expressed as full-cycle visibilities per second
+    _CPU["S"] = S
+    facets = sorted(set(int(x) for x in np.linspace(0, nf - 1, min(cores, nf)).round()))
+    ctx = mp.get_context("fork")
+    with ctx.Pool(len(facets)) as pool:
+        t0 = time.perf_counter()
+        pool.map(_cpu_facet_job, facets, chunksize=1)
+        t = time.perf_counter() - t0
+    _CPU.clear()
     vf_per_s = nv * len(facets) / t
-    return dict(value=vf_per_s / nf, unit="vis/s", cores=1, kind="reference",
+    return dict(value=vf_per_s / nf, unit="vis/s", cores=len(facets), kind="reference",
                 sample="reference _pyGridderSmearPols3x (oracle/_ref, built from DDFacet/Gridder sources), "
-                       "grid+degrid of every %d-th BDA block (%d vis) for facets %s, 1 core, %.1f s; "
-                       "value = vis-facet pairs/s / %d facets" % (stride, nv, facets, t, nf))
+                       "%d worker processes (one facet each, facets %s), grid + degrid of every %d-th BDA "
+                       "block (%d vis), %.1f s wall; value = vis-facet pairs/s / %d facets"
+                       % (len(facets), facets, stride, nv, t, nf))
+
+
+def time_graph(fs, steps):
+    """Mean ms of one batched cycle replayed from a HIP graph (capture excluded)."""
+    import torch
+    g = fs.capture_cycle()
+    g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.replay()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def run_uneven(a):
+    """121 facets x 4 bands, baseline-dependent averaging: plain per-facet launches against the
+    autotuned batched cycle over the same major cycle (grid all facets, degrid all facets)."""
+    import torch
+    from ddfacet_amd import autotune, configs
+    from ddfacet_amd.facets import DeviceFacetSet
+    P = configs.cfg4(log=log)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    fs = DeviceFacetSet(P, device=dev)
+    gen = torch.Generator(device=dev).manual_seed(7)
+    fs.models.copy_(torch.randn(fs.models.shape, generator=gen, device=dev, dtype=torch.complex64) * 1e-3)
+    vis0 = fs.t["vis"].clone()
+    s = torch.cuda.current_stream()
+    R, C = P.uvw.shape[0], P.freqs.shape[0]
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        fs.t["vis"].copy_(vis0)
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    def plain():
+        fs.zero_grids()
+        fs.grid_plain(stream=s)
+        fs.degrid_plain(stream=s)
+
+    def batched():
+        fs.zero_grids()
+        fs.grid_batched(stream=s)
+        fs.degrid_batched(stream=s)
+
+    ms_plain = timed(plain, a.steps, a.warmup)
+    log("plain: %.2f ms per cycle" % ms_plain)
+    space = {"tile": (16, 32, 64), "chunk": (1024, 4096), "blocks_per_item": (32, 128)}
+    best, results = autotune.autotune(fs, space=space, reps=2, log=log)
+    ms_batched = timed(batched, a.steps, a.warmup)
+    ms_graph = time_graph(fs, a.steps)
+    fs.t["vis"].copy_(vis0)
+    info = fs.plan_info()
+    out = {
+        "metric": "vis/sec grid+degrid, 121-facet uneven BDA, 4 bands",
+        "value": R * C / (ms_batched * 1e-3), "unit": "vis/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms_batched, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 (fp32 grids, fp64 phase base)", "data": "synthetic",
+        "config": {"workload": "121 facets (11 x 11, 567 -> 975 px at 2\"), 4 bands, 27 antennas 20 km disk, "
+                               "100 x 10 s, 16 ch, baseline-dependent averaging 1-16 by length quintile",
+                   "rows": R, "channels": C, "nvis": R * C, "facets": len(P.facets),
+                   "rows_per_factor_1_2_4_8_16": P.extra["rows_per_factor"],
+                   "grid_records": info["grid_records"], "grid_items": info["grid_items"]},
+        "plain_ms_per_step": ms_plain, "plain_value": R * C / (ms_plain * 1e-3),
+        "speedup_vs_plain": ms_plain / ms_batched, "graph_ms_per_step": ms_graph,
+        "autotune": {"best": best, "candidates": [[c, ms] for c, ms in results]},
+    }
+    print(json.dumps(out))
+
+
+def run_autograd(a, gd_steps=20):
+    """chi^2 gradient descent through the differentiable degridder (forward: model cube ->
+    visibilities, backward: the hand-written adjoint), 25 facets x 4 bands."""
+    import torch
+    from ddfacet_amd import configs
+    from ddfacet_amd.autograd import DegridOperator, chi2
+    from ddfacet_amd.cycle import MajorCycle
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    P, geom = configs.cfg5(device="cuda", log=log)
+    cyc = MajorCycle(P, geom)
+    op = DegridOperator(cyc)
+    data = P.vis
+    w = torch.from_numpy(P.weights).to(dev)
+    shape = (P.nband, P.npol, geom.m, geom.m)
+    gen = torch.Generator(device=dev).manual_seed(5003)
+    # adjoint check on random vectors
+    xr = torch.randn(shape, generator=gen, device=dev)
+    yr = torch.randn(data.shape, generator=gen, device=dev, dtype=torch.complex64)
+    lhs = torch.sum(op.forward(xr).double().conj() * yr.to(torch.complex128)).real
+    rhs = torch.sum(xr.double() * op.adjoint(yr).double())
+    dot_rel = float(abs(lhs - rhs) / max(abs(float(lhs)), 1e-30))
+    # step 1 / L, L = 2 lambda_max(A^H W A) by power iteration
+    y = xr / xr.norm()
+    lam = 0.0
+    for _ in range(8):
+        z = op.adjoint(w[..., None] * op.forward(y))
+        lam = float(z.norm())
+        y = z / lam
+    alpha = 1.0 / (2.0 * lam)
+    x = torch.zeros(shape, device=dev, requires_grad=True)
+
+    def gd_step():
+        loss = chi2(x, op, data, w)
+        loss.backward()
+        with torch.no_grad():
+            x -= alpha * x.grad
+        x.grad = None
+        return loss
+
+    first = float(gd_step())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(gd_steps):
+        last = gd_step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / gd_steps
+    last = float(last)
+    R, C = P.uvw.shape[0], P.freqs.shape[0]
+    out = {
+        "metric": "chi^2 gradient-descent steps through the differentiable degridder, 25 facets x 4 bands",
+        "value": R * C / (ms * 1e-3), "unit": "vis/s (one forward + one adjoint per step)", "n_gpus": 1,
+        "steps": gd_steps, "warmup": 1, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 (fp32 grids, fp64 phase base)", "data": "synthetic",
+        "config": {"workload": "25 facets (5 x 5, 405 -> 693 px at 2\"), model cube 4 x 2025 x 2025, 27 antennas "
+                               "20 km disk, 300 x 10 s, 32 ch in 4 bands, 100 sources + noise 0.05",
+                   "rows": R, "channels": C, "nvis": R * C},
+        "chi2_first": first, "chi2_after": last, "dot_product_rel_mismatch": dot_rel, "step_size": alpha,
+    }
+    print(json.dumps(out))
 
 
 def main():
@@ -125,9 +320,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="plain", choices=["plain", "batched"])
+    ap.add_argument("--mode", default="batched", choices=["batched", "plain"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--facets", type=int, default=0, help="use only the first N facets (debug)")
+    ap.add_argument("--workload", default="meerkat81", choices=["meerkat81", "uneven121", "autograd25"],
+                    help="meerkat81: the 81-facet MeerKAT-like cycle (the headline line); uneven121: "
+                         "121 facets x 4 bands with baseline-dependent averaging, plain per-facet "
+                         "launches against the autotuned batched cycle (eager and graph-replayed); "
+                         "autograd25: 20 chi^2 gradient-descent steps through the differentiable "
+                         "degridder on 25 facets x 4 bands")
+    ap.add_argument("--graph", action="store_true",
+                    help="also time the batched cycle replayed from a HIP graph")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a device copy of 1 GiB (reads 1 GiB, writes 1 GiB), to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiler run")
@@ -147,15 +350,28 @@ def main():
               "the path would shard); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    import torch
+    if a.workload == "uneven121":
+        return run_uneven(a)
+    if a.workload == "autograd25":
+        return run_autograd(a)
     from ddfacet_amd import configs
+    t0 = time.time()
+    # geometry, maps, CFs and Jones on the host; the CPU baseline runs before the GPU is touched
+    P = configs.cfg2(with_vis=False, log=log)
+    if a.facets:
+        P.facets = P.facets[:a.facets]
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline(P)
+            log("cpu baseline: %s" % (cpu,))
+        except Exception as e:  # the baseline must not hide the GPU result
+            log("cpu baseline failed: %r" % (e,))
+    import torch
     from ddfacet_amd.facets import DeviceFacetSet
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    t0 = time.time()
-    P = configs.cfg2(device="cuda", log=log)
-    if a.facets:
-        P.facets = P.facets[:a.facets]
+    P.vis = configs.cfg2_visibilities(P.uvw, P.freqs, P.extra["sky"], device="cuda")
     fs = DeviceFacetSet(P, device=dev)
     nf = fs.nf
     gen = torch.Generator(device=dev).manual_seed(7)
@@ -164,7 +380,13 @@ def main():
     log("setup %.1fs" % (time.time() - t0))
 
     stream = torch.cuda.current_stream()
+    if a.mode == "batched":
+        fs.plan_batched(stream=stream)
+        fs.plan_timing(True)
+        info = fs.plan_info()
+        log("plan: %s" % info)
     ev = {"grid": [], "degrid": []}
+    passes = []
 
     def step(record):
         fs.zero_grids()
@@ -194,6 +416,8 @@ def main():
     t1 = time.perf_counter()
     for _ in range(a.steps):
         step(True)
+        if a.mode == "batched":
+            passes.append(fs.pass_ms())  # waits for this step's pass events
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     ms = (t2 - t1) * 1e3 / a.steps
@@ -201,36 +425,49 @@ def main():
     R, C = P.uvw.shape[0], P.freqs.shape[0]
     nvis = R * C
     value = nvis / (ms * 1e-3)
-    tg = np.mean([x.elapsed_time(y) for x, y in ev["grid"]])
-    td = np.mean([x.elapsed_time(y) for x, y in ev["degrid"]])
-    launches = nf if a.mode == "plain" else 1
-    per = {"grid": tg / launches, "degrid": td / launches}
-    dom = "grid" if tg >= td else "degrid"
-    nfl = 1 if a.mode == "plain" else nf
-    ach = algorithmic_bytes(P, nfl, dom) / (per[dom] * 1e-3)
+    tg = float(np.mean([x.elapsed_time(y) for x, y in ev["grid"]]))
+    td = float(np.mean([x.elapsed_time(y) for x, y in ev["degrid"]]))
     log("grid %.2f ms, degrid %.2f ms per cycle (%s)" % (tg, td, a.mode))
-    cpu = None
-    if not a.no_cpu:
-        try:
-            cpu = cpu_baseline(P)
-        except Exception as e:  # the baseline must not hide the GPU result
-            log("cpu baseline failed: %r" % (e,))
+    if a.mode == "batched":
+        ns = len(P.stokes_codes)
+        pms = {k: float(np.mean([d[k] for d in passes])) for k in passes[0] if passes[0][k] is not None}
+        pb = pass_bytes(P, nf, info, ns)
+        dom = max(pms, key=pms.get)
+        launch_ms, abytes = pms[dom], pb[dom]
+        extra = {"passes_ms": pms,
+                 "passes_gbps": {k: pb[k] / (pms[k] * 1e-3) / 1e9 for k in pms if k in pb}}
+    else:
+        dom = "grid_plain" if tg >= td else "degrid_plain"
+        launch_ms = (tg if dom == "grid_plain" else td) / nf
+        abytes = plain_bytes(P, "grid" if dom == "grid_plain" else "degrid")
+        extra = {}
+    ach = abytes / (launch_ms * 1e-3)
+    cb = cycle_bytes(P, nf)
+    graph_ms = None
+    if a.graph and a.mode == "batched":
+        graph_ms = time_graph(fs, a.steps)
+        fs.t["vis"].copy_(vis0)
+    cfg = {"workload": "81-facet MeerKAT-like: 9x9 facets (455 -> 735 px at 1.5\"), 64 antennas, "
+                       "600 x 8 s, 64 ch 0.9-1.67 GHz, 4 corr, per-visibility DD Jones, default BDA (0.02)",
+           "rows": R, "channels": C, "nvis": nvis, "facets": nf,
+           "bda_blocks": fs.nblocks_grid, "mode": a.mode, "grid_ms": tg, "degrid_ms": td}
+    if a.mode == "batched":
+        cfg.update(grid_records=info["grid_records"], degrid_records=info["degrid_records"])
     out = {
         "metric": "vis/sec grid+degrid, 81-facet MeerKAT-like",
         "value": value, "unit": "vis/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64 (fp32 grids, fp64 phase base)", "data": "synthetic",
-        "config": {"workload": "cfg2: 81 facets (9x9, 455->735 px at 1.5\"), 64 ant MeerKAT-like, "
-                               "600x8s, 64 ch 0.9-1.67 GHz, 4 corr, per-vis DD Jones, default BDA (0.02)",
-                   "rows": R, "channels": C, "nvis": nvis, "facets": nf,
-                   "bda_blocks": fs.nblocks_grid, "mode": a.mode, "grid_ms": tg, "degrid_ms": td},
-        "roofline": {"bound": "hbm", "kernel": ("grid_plain" if dom == "grid" else "degrid_plain")
-                     if a.mode == "plain" else dom + "_batched",
-                     "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK, "traffic": None,
-                     "launch_ms": per[dom], "algorithmic_bytes_per_launch": algorithmic_bytes(P, nfl, dom)},
+        "config": cfg,
+        "roofline": dict({"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9,
+                          "unit": "GB/s", "frac": ach / HBM_PEAK, "traffic": None, "launch_ms": launch_ms,
+                          "algorithmic_bytes_per_launch": abytes,
+                          "cycle": {"bytes": cb, "ms": ms, "achieved": cb / (ms * 1e-3) / 1e9,
+                                    "frac": cb / (ms * 1e-3) / HBM_PEAK}}, **extra),
         "cpu_baseline": cpu,
     }
+    if graph_ms is not None:
+        out["graph_ms_per_step"] = graph_ms
     print(json.dumps(out))
 
 

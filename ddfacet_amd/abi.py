@@ -163,6 +163,8 @@ EXTRA_SIGNATURES = {
     "ddf_plan_grid": [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p],
     "ddf_plan_degrid": [C.c_void_p, C.c_void_p, C.c_void_p],
     "ddf_plan_info": [C.c_void_p, C.POINTER(C.c_int64), C.c_int32],
+    "ddf_plan_timing": [C.c_void_p, C.c_int32],
+    "ddf_plan_pass_ms": [C.c_void_p, C.POINTER(C.c_float), C.c_int32],
     "ddf_plan_destroy": [C.c_void_p],
     "ddf_plan_degrid_adjoint": [C.c_void_p, C.c_void_p, C.c_void_p],
     "ddf_model_to_grid_adjoint": [C.POINTER(ddf_image_facet), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -174,3 +176,6 @@ EXTRA_SIGNATURES = {
     "ddf_model_to_grid": [C.POINTER(ddf_image_facet), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                           C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p],
 }
+
+# pass order of ddf_plan_pass_ms (DDF_PASS_* in include/ddfgpu.h)
+PASS_NAMES = ("grid_pass_a", "grid_pass_b", "degrid_pass_a", "degrid_pass_b", "degrid_adj_b", "degrid_adj_a")

@@ -3,6 +3,19 @@
 cfg1  plain single facet: 27 antennas in a 20 km disk, 351 baselines, 100 x 10 s, 16 channels
       1.2-1.6 GHz, 512 x 512 grid at 2", Nw = 8, S = 7, OverS = 11, identity Jones, one BDA
       block per (row, channel), 10 point sources + noise, 1% flags.
+cfg3  precision study: the cfg1 array with 256 channels over 0.9-1.67 GHz, one facet of 301 px at
+      2" padded to 512 (Nw = 8, S = 7, OverS = 11), 20 point sources at pixel centres 4-12 px
+      inside the facet edges, visibilities from the FP64 DFT, no noise, no flags.
+cfg4  uneven batched workload: the cfg1 array (27 antennas, 20 km disk, 100 x 10 s, 16 channels
+      1.2-1.6 GHz) with baseline-dependent averaging k_t = k_f in {1, 2, 4, 8, 16} by baseline-length
+      quintile (longest -> 1, shortest -> 16; blocks cut at the 4-channel band edges, so the
+      frequency factor is at most 4), NBand = 4 sub-band grids, 121 facets (11 x 11) of 567 px
+      padded to 975 at 2" on a 6237 px image, Nw = 8, S = 7, OverS = 11, no Jones, 10 point
+      sources + noise (seeds 4000 + k).
+cfg5  differentiable forward model: the cfg1 array at 300 x 10 s, 32 channels 1.2-1.6 GHz in 4
+      bands, 25 facets (5 x 5) of 405 px padded to 693 at 2" (image 2025 px), Nw = 8, S = 7,
+      OverS = 11, one BDA block per visibility, 100 point sources at pixel centres (log-uniform
+      0.01-1 Jy) + complex noise sigma 0.05 (seeds 5000 + k).
 cfg2  81-facet MeerKAT-like: 64 antennas (dense 1 km core + 3 arms to 8 km), 2016 baselines,
       600 x 8 s, 64 channels 0.9-1.67 GHz, 4096 x 4096 image at 1.5" in 9 x 9 facets of 455 px
       padded to 735 (EstimateNpix with Padding 1.7), Nw = 32 with supports odd(linspace(7, 15)),
@@ -125,12 +138,7 @@ def cfg2(nfx=9, image_npix=4096, ntime=600, nchan=64, nant=64, sort_by_baseline=
     sky = synth.sky_powerlaw(200, 0.5 * image_npix * cell, rng)
     vis = None
     if with_vis:
-        import torch
-        v = synth.predict_point_sources(obs.uvw, freqs, sky, device=device, chunk_rows=4096)
-        g = torch.Generator(device=device).manual_seed(2002)
-        noise = torch.randn(v.shape, generator=g, device=device, dtype=torch.complex64) * 0.01
-        v += noise
-        vis = v if device != "cpu" else v.numpy()
+        vis = cfg2_visibilities(obs.uvw, freqs, sky, device)
         log("cfg2: visibilities in %.1fs" % (time.time() - t0))
     flags = synth.random_flags(R, nchan, 4, 0.01, 2003)
     weights = np.ones((R, nchan), np.float32)
@@ -141,6 +149,16 @@ def cfg2(nfx=9, image_npix=4096, ntime=600, nchan=64, nant=64, sort_by_baseline=
     P.extra.update(sky=sky, cell_arcsec=cell_as, image_npix=image_npix, facet_npix=nopt, name="cfg2",
                    nfx=nfx, equidistant=chan_equidistant(freqs))
     return P
+
+
+def cfg2_visibilities(uvw, freqs, sky, device="cpu"):
+    """The 81-facet workload's visibilities: FP32 point-source predict plus complex Gaussian
+    noise sigma 0.01 (seed 2002), on `device` (a torch tensor there, numpy on the CPU)."""
+    import torch
+    v = synth.predict_point_sources(uvw, freqs, sky, device=device, chunk_rows=4096)
+    g = torch.Generator(device=device).manual_seed(2002)
+    v += torch.randn(v.shape, generator=g, device=device, dtype=torch.complex64) * 0.01
+    return v if device != "cpu" else v.numpy()
 
 
 def small_cycle(seed=5, nfx=3, m=243, n=147, cell_arcsec=20.0, nant=7, ntime=10, nchan=6, nsrc=6,
@@ -188,4 +206,131 @@ def small_cycle(seed=5, nfx=3, m=243, n=147, cell_arcsec=20.0, nant=7, ntime=10,
                 facets=facets, npix=n, nband=1, npol=1, incr=np.array([-cell, cell]),
                 corr_codes=[9, 10, 11, 12], stokes_codes=[1], jones=jd)
     P.extra.update(sky=sky, src_pix=(X, Y), cell=cell, image_npix=m, name="small_cycle")
+    return P, geom
+
+
+def cfg3(nsrc=20, m=301, n=512, nchan=256, cell_arcsec=2.0, log=print):
+    """The precision-study workload (see the module docstring).  Returns (Problem, geometry)."""
+    from .imaging import FacetGeometry
+    t0 = time.time()
+    freqs = np.linspace(0.9e9, 1.67e9, nchan)
+    obs = synth.make_observation("disk", 27, 100, 10.0, freqs, 45.0, -30.7, 3000, diameter_m=20000.0)
+    cell = cell_arcsec * synth.ARCSEC
+    geom = FacetGeometry(m, 1, m, n, cell, mixing_width=4.0)
+    rng = np.random.default_rng(3001)
+    side = rng.integers(0, 4, nsrc)
+    along = rng.integers(12, m - 12, nsrc)
+    inset = rng.integers(4, 13, nsrc)
+    X = np.where(side == 0, inset, np.where(side == 1, m - 1 - inset, along))
+    Y = np.where(side == 2, inset, np.where(side == 3, m - 1 - inset, along))
+    flux = rng.uniform(0.5, 2.0, nsrc)
+    sky = synth.Sky(flux, (X - m // 2) * cell, (Y - m // 2) * cell)
+    vis = synth.predict_point_sources(obs.uvw, freqs, sky).numpy()
+    R = obs.nrow
+    flags = np.zeros((R, nchan, 4), np.uint8)
+    weights = np.ones((R, nchan), np.float32)
+    wmax = float(np.abs(obs.uvw[:, 2]).max()) + 1e-3
+    f = make_cf_facet(0.0, 0.0, n, cell_arcsec, freqs, 8, [7] * 8, 11, wmax)
+    f.kms_dir = 0
+    bmap = bdamod.bda_map_per_visibility(obs.a0, obs.a1, nchan)
+    chan_map = np.zeros(nchan, np.int32)
+    P = Problem(uvw=obs.uvw, vis=vis, flags=flags, weights=weights, freqs=freqs, a0=obs.a0, a1=obs.a1,
+                chan_map=chan_map, chan_map_degrid=chan_map.copy(), bda_grid=bmap, bda_degrid=bmap,
+                facets=[f], npix=n, nband=1, npol=1, incr=np.array([-cell, cell]),
+                corr_codes=[9, 10, 11, 12], stokes_codes=[1])
+    P.extra.update(sky=sky, src_pix=(X, Y), cell=cell, image_npix=m, name="cfg3",
+                   equidistant=chan_equidistant(freqs))
+    log("cfg3 built in %.1fs: %d rows x %d chan" % (time.time() - t0, R, nchan))
+    return P, geom
+
+
+def cfg4(nfx=11, image_npix=6144, cell_arcsec=2.0, nband=4, noise=0.01, log=print):
+    """The uneven batched workload (see the module docstring)."""
+    t0 = time.time()
+    nchan = 16
+    freqs = np.linspace(1.2e9, 1.6e9, nchan)
+    obs = synth.make_observation("disk", 27, 100, 10.0, freqs, 45.0, -30.7, 4000, diameter_m=20000.0,
+                                 sort_by_baseline=True)
+    cell = cell_arcsec * synth.ARCSEC
+    nopt, npad = estimate_npix(image_npix / nfx, 1.7)
+    # averaging factor per baseline by length quintile (longest -> 1, shortest -> 16)
+    keys = ["%d:%d" % k for k in zip(obs.a0.tolist(), obs.a1.tolist())]
+    bl_len = {}
+    for k, q in zip(keys, np.hypot(obs.uvw[:, 0], obs.uvw[:, 1])):
+        bl_len.setdefault(k, []).append(q)
+    lens = {k: float(np.mean(v)) for k, v in bl_len.items()}
+    edges = np.quantile(np.array(list(lens.values())), [0.2, 0.4, 0.6, 0.8])
+    factors = (16, 8, 4, 2, 1)
+    fac = {k: factors[int(np.searchsorted(edges, L, side="right"))] for k, L in lens.items()}
+    chan_map = (np.arange(nchan) * nband // nchan).astype(np.int32)
+    bmap = bdamod.bda_map_factors(obs.a0, obs.a1, nchan, lambda key: (fac[key], fac[key]), chan_map=chan_map)
+    wmax = float(np.abs(obs.uvw[:, 2]).max()) + 1e-3
+    cen = synth.facet_centres(nfx, nopt, cell)
+    facets = []
+    for i in range(nfx * nfx):
+        f = make_cf_facet(float(cen[i, 0]), float(cen[i, 1]), npad, cell_arcsec, freqs, 8, [7] * 8, 11, wmax)
+        f.kms_dir = i
+        facets.append(f)
+    rng = np.random.default_rng(4001)
+    sky = synth.sky_loguniform(10, 0.45 * nfx * nopt * cell, rng)
+    vis = synth.predict_point_sources(obs.uvw, freqs, sky).numpy()
+    nr = np.random.default_rng(4002)
+    vis += (noise / np.sqrt(2)) * (nr.standard_normal(vis.shape) + 1j * nr.standard_normal(vis.shape)).astype(np.complex64)
+    R = obs.nrow
+    flags = synth.random_flags(R, nchan, 4, 0.01, 4003)
+    weights = np.ones((R, nchan), np.float32)
+    P = Problem(uvw=obs.uvw, vis=vis, flags=flags, weights=weights, freqs=freqs, a0=obs.a0, a1=obs.a1,
+                chan_map=chan_map, chan_map_degrid=chan_map.copy(), bda_grid=bmap, bda_degrid=bmap,
+                facets=facets, npix=npad, nband=nband, npol=1, incr=np.array([-cell, cell]),
+                corr_codes=[9, 10, 11, 12], stokes_codes=[1])
+    counts = np.bincount(np.array([fac[k] for k in keys]), minlength=17)[[1, 2, 4, 8, 16]]
+    P.extra.update(sky=sky, cell_arcsec=cell_arcsec, image_npix=nfx * nopt, facet_npix=nopt, name="cfg4",
+                   nfx=nfx, equidistant=chan_equidistant(freqs), rows_per_factor=counts.tolist())
+    log("cfg4: %d rows x %d chan, %d facets %d -> %d px, %d BDA blocks, built in %.1fs"
+        % (R, nchan, nfx * nfx, nopt, npad, bdamod.nblocks_of(bmap), time.time() - t0))
+    return P
+
+
+def cfg5(nfx=5, image_npix=2048, cell_arcsec=2.0, nband=4, ntime=300, nchan=32, nsrc=100, noise=0.05,
+         device="cpu", log=print):
+    """The differentiable forward-model workload (see the module docstring).  Returns
+    (Problem, geometry); the visibilities live on `device`."""
+    from .imaging import FacetGeometry
+    t0 = time.time()
+    freqs = np.linspace(1.2e9, 1.6e9, nchan)
+    obs = synth.make_observation("disk", 27, ntime, 10.0, freqs, 45.0, -30.7, 5000, diameter_m=20000.0)
+    cell = cell_arcsec * synth.ARCSEC
+    nopt, npad = estimate_npix(image_npix / nfx, 1.7)
+    m = nfx * nopt
+    geom = FacetGeometry(m, nfx, nopt, npad, cell, mixing_width=10.0)
+    rng = np.random.default_rng(5001)
+    X = rng.integers(m // 8, m - m // 8, nsrc)
+    Y = rng.integers(m // 8, m - m // 8, nsrc)
+    flux = np.exp(rng.uniform(np.log(0.01), np.log(1.0), nsrc))
+    sky = synth.Sky(flux, (X - m // 2) * cell, (Y - m // 2) * cell)
+    import torch
+    v = synth.predict_point_sources(obs.uvw, freqs, sky, device=device, chunk_rows=8192)
+    g = torch.Generator(device=device).manual_seed(5002)
+    v += torch.randn(v.shape, generator=g, device=device, dtype=torch.complex64) * noise
+    vis = v if device != "cpu" else v.numpy()
+    R = obs.nrow
+    flags = np.zeros((R, nchan, 4), np.uint8)
+    weights = np.ones((R, nchan), np.float32)
+    wmax = float(np.abs(obs.uvw[:, 2]).max()) + 1e-3
+    facets = []
+    for i in range(geom.nf):
+        l0, m0 = geom.lm[i]
+        f = make_cf_facet(l0, m0, npad, cell_arcsec, freqs, 8, [7] * 8, 11, wmax)
+        f.kms_dir = i
+        facets.append(f)
+    chan_map = (np.arange(nchan) * nband // nchan).astype(np.int32)
+    bmap = bdamod.bda_map_per_visibility(obs.a0, obs.a1, nchan)
+    P = Problem(uvw=obs.uvw, vis=vis, flags=flags, weights=weights, freqs=freqs, a0=obs.a0, a1=obs.a1,
+                chan_map=chan_map, chan_map_degrid=chan_map.copy(), bda_grid=bmap, bda_degrid=bmap,
+                facets=facets, npix=npad, nband=nband, npol=1, incr=np.array([-cell, cell]),
+                corr_codes=[9, 10, 11, 12], stokes_codes=[1])
+    P.extra.update(sky=sky, src_pix=(X, Y), cell=cell, image_npix=m, name="cfg5",
+                   equidistant=chan_equidistant(freqs))
+    log("cfg5: %d rows x %d chan, %d facets %d -> %d px, image %d, built in %.1fs"
+        % (R, nchan, geom.nf, nopt, npad, m, time.time() - t0))
     return P, geom

@@ -16,7 +16,8 @@
 //   grid B  persistent over bins: the S x S stamps of a bin accumulate in an LDS tile (T + 2 h)^2,
 //           flushed once into the facet grid with global float atomics (non-zero cells only).
 //   degrid A persistent over bins: the model tile is staged in LDS; one wavefront gathers each
-//           record's stamp (FP32 products summed in FP64, degridder.h:202-212) -> M[b][f].
+//           record's stamp (FP32 products summed in FP64 with a DPP wave reduction,
+//           degridder.h:202-212) -> M[b][f].
 //   degrid B lanes are visibilities; each lane sums the model of every facet (Stokes -> corr,
 //           Jones, phasor, decorrelation) in registers and writes its visibility once.
 // Work queues are device counters reset by a small kernel (no memset nodes, capture-safe).
@@ -183,9 +184,15 @@ __global__ __launch_bounds__(256) void plan_gather_place(const uint64_t* place, 
     bplace[k] = place[recs[k]];
 }
 
-// Jones table [t][dir][ant][chan][4] -> [t][ant][chan][dir][4]
-__global__ __launch_bounds__(256) void plan_transpose_jones(const float2* src, float2* dst, int nt, int nd, int na,
-                                                            int nch) {
+// Prepared Jones table of the batched plan: entry (t, dir, ant, chan) of a raw table
+// [t][dir][ant][chan][4] through GiveJones (interpolation mode) and, for the DD table, the
+// amplitude / phase part of NormJones, in FP64 at [t][ant][chan][dir][4] (facet-fastest: the
+// lanes of a wavefront are facets).  ScaleJones depends on the visibility and stays in the passes.
+__global__ __launch_bounds__(256) void plan_prepare_jones(const float2* src, double2* dst, JonesP jp, int dd) {
+  const int* dims = dd ? jp.kdims : jp.bdims;
+  const int nt = dims[0], nd = dims[1], na = dims[2], nch = dims[3];
+  JonesP q = jp;
+  q.scale = 0;
   const int64_t total = int64_t(nt) * nd * na * nch;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x) {
     int64_t r = i;
@@ -193,9 +200,11 @@ __global__ __launch_bounds__(256) void plan_transpose_jones(const float2* src, f
     const int a = int(r % na); r /= na;
     const int d = int(r % nd); r /= nd;
     const int t = int(r);
+    M2 J = give_jones(src, dims, t, a, d, ch, jp.interp, jp.coefs, jp.ncoefs);
+    if (dd) norm_jones(q, J, nullptr, 0.);
     const int64_t o = ((int64_t(t) * na + a) * nch + ch) * nd + d;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) dst[o * 4 + q] = src[i * 4 + q];
+    for (int k = 0; k < 4; ++k) dst[o * 4 + k] = make_double2(J.a[k].re, J.a[k].im);
   }
 }
 
@@ -234,6 +243,8 @@ __global__ void reset_counter(uint32_t* c, int n) {
 
 // ---- cycle: grid pass A --------------------------------------------------------------------------
 // One workgroup handles one facet group; lane = slot * G + facet-in-group.
+// JMODE: 0 no Jones, 1 per-visibility Jones (applied once per run of one Jones key), 2 per-block
+// Jones, 3 per-visibility Jones with ScaleJones (applied per visibility).
 template <int READ, int JMODE, int PREC>
 __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, const uint64_t* place, float2* V,
                                                    uint32_t* queue) {
@@ -322,17 +333,104 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
           }
         }
         double uvwm[3] = {su / cnt, sv / cnt, sw / cnt};
-        jblk = jones_at(jp, mid_ok ? mid : 0, (br.ch0 + br.ch1) / 2, uvwm, p.wl_mean);
+        jblk = jones_at<true>(jp, mid_ok ? mid : 0, (br.ch0 + br.ch1) / 2, uvwm, p.wl_mean);
         BBblk = jblk.BB;
       }
       double* sch = s_chan + lf * 2 * v.nchan;
-      JonesCache jc;
+      if (JMODE == 1) {
+        // Per-visibility Jones without ScaleJones depends only on the key (baseline, Jones time,
+        // Jones channel), so J0^H (sum_v meas_v w_v) J1 over a run of one key equals the sum of
+        // the per-visibility products (gridder.h:255-260): the block sum is accumulated per run
+        // and the Jones pair applied once per run (rows and channels in the reference order).
+        cd racc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        double rfw = 0;  // PSF: sum of the run's weights
+        int kt = -1, kc = -1, bt = -1, bc = -1;
+        JonesPair cur;
+        cur.BB = 0;
+        auto flush = [&]() {
+          M2 m;
+          if (p.dopsf) {
+            m = mul(mul(herm(cur.J0), mul(cur.J0, herm(cur.J1))), cur.J1);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) m.a[q] = m.a[q] * (dec * rfw);
+          } else {
+            m = mul(mul(herm(cur.J0), M2{{racc[0], racc[1], racc[2], racc[3]}}), cur.J1);
+          }
+          if (READ == 4) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = acc[q] + m.a[q];
+          } else if (READ == 2) {
+            acc[0] = acc[0] + m.a[0];
+            acc[3] = acc[3] + m.a[3];
+          } else {
+            acc[0] = acc[0] + m.a[0];
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) racc[q] = cd{0, 0};
+          rfw = 0;
+        };
+        bool any = false;
+        for (int r = 0; r < br.nr; ++r) {
+          const int64_t row = br.rows[r];
+          if (row < 0 || row >= v.nrow) continue;
+          const double* uvw = v.uvw + 3 * row;
+          ChanPhasor<PREC> ph;
+          ph.row(-2. * kPi * (uvw[0] * fd.l0 + uvw[1] * fd.m0 + uvw[2] * fd.n0) / kC);
+          const int k_t = jp.kms ? jp.ktime[row] : -1;
+          const int b_t = jp.beam ? jp.btime[row] : -1;
+          for (int c = br.ch0; c < br.ch1; ++c) {
+            const size_t vc = size_t(row) * v.nchan + c;
+            if (v.flags[vc * v.ncorr]) continue;
+            const int k_c = jp.kms ? jp.kchan[c] : -1, b_c = jp.beam ? jp.bchan[c] : -1;
+            if (!any || k_t != kt || k_c != kc || b_t != bt || b_c != bc) {
+              if (any) flush();
+              cur = jones_at<true>(jp, row, c, uvw, p.wl_mean);
+              kt = k_t; kc = k_c; bt = b_t; bc = b_c;
+              any = true;
+            }
+            const double fw = double(v.weights[vc]);
+            const double fwd = fw * dec * dec;
+            if (p.dopsf) {
+              rfw += fw;
+            } else {
+              const cd wt = ph.at(v, c) * fw;
+              const float2* vp = v.vis + vc * v.ncorr;
+              if (READ == 4) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) racc[q] = racc[q] + cd{vp[q].x, vp[q].y} * wt;
+              } else if (READ == 2) {
+                racc[0] = racc[0] + cd{vp[0].x, vp[0].y} * wt;
+                racc[3] = racc[3] + cd{vp[1].x, vp[1].y} * wt;
+              } else {
+                const cd x = cd{vp[0].x, vp[0].y} * wt;
+                racc[0] = racc[0] + x;
+                racc[3] = racc[3] + x;
+              }
+            }
+            const double bbw = cur.BB * fwd;
+            sj += bbw;
+            const double x0 = double(npg) * bbw, x1 = double(npg) * fwd;
+            if (p.chan_lds) {
+              atomicAdd(&sch[c], x0);
+              atomicAdd(&sch[v.nchan + c], x1);
+            } else {
+              atomicAdd(&fd.sum_jones_chan[c], x0);
+              atomicAdd(&fd.sum_jones_chan[v.nchan + c], x1);
+            }
+            swt += fw;
+            ssw += fwd;
+          }
+        }
+        if (any) flush();
+      } else {
+      JonesCache<true> jc;
       jc.reset();
       for (int r = 0; r < br.nr; ++r) {
         const int64_t row = br.rows[r];
         if (row < 0 || row >= v.nrow) continue;
         const double* uvw = v.uvw + 3 * row;
-        const double angle = -2. * kPi * (uvw[0] * fd.l0 + uvw[1] * fd.m0 + uvw[2] * fd.n0) / kC;
+        ChanPhasor<PREC> ph;
+        ph.row(-2. * kPi * (uvw[0] * fd.l0 + uvw[1] * fd.m0 + uvw[2] * fd.n0) / kC);
         for (int c = br.ch0; c < br.ch1; ++c) {
           const size_t vc = size_t(row) * v.nchan + c;
           if (v.flags[vc * v.ncorr]) continue;
@@ -353,8 +451,8 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
             }
           }
           cd wt = cd{fw, 0};
-          if (!p.dopsf) wt = phasor_prec<PREC>(v.freqs[c], angle) * fw;
-          if (JMODE == 1) {
+          if (!p.dopsf) wt = ph.at(v, c) * fw;
+          if (JMODE == 1 || JMODE == 3) {
             const JonesPair& jj = jc.get(jp, row, c, uvw, p.wl_mean);
             M2 m;
             if (p.dopsf) {
@@ -394,6 +492,7 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
           ssw += fwd;
         }
       }
+      }
       if (JMODE == 2) {
         M2 m{{acc[0], acc[1], acc[2], acc[3]}};
         if (p.dopsf) m = mul(mul(jblk.J0, m), herm(jblk.J1));
@@ -432,30 +531,104 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
   }
 }
 
-// ---- cycle: grid pass B (LDS tiles) ---------------------------------------------------------------
+// ---- wave-level helpers for the tile passes ---------------------------------------------------------
+// Values of lane j (j uniform) in scalar registers.
+__device__ __forceinline__ uint32_t lane_u32(uint32_t x, int j) { return uint32_t(__builtin_amdgcn_readlane(int(x), j)); }
+__device__ __forceinline__ uint64_t lane_u64(uint64_t x, int j) {
+  return uint64_t(lane_u32(uint32_t(x), j)) | (uint64_t(lane_u32(uint32_t(x >> 32), j)) << 32);
+}
+__device__ __forceinline__ float lane_f32(float x, int j) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j)); }
+__device__ __forceinline__ int uniform_i32(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int64_t uniform_i64(int64_t x) {
+  return int64_t(uint64_t(uint32_t(uniform_i32(int(x)))) | (uint64_t(uint32_t(uniform_i32(int(x >> 32)))) << 32));
+}
+
+// One DPP step of a 64-bit value: lanes of rows outside ROWMASK receive 0.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const uint64_t b = uint64_t(__double_as_longlong(x));
+  const int lo = __builtin_amdgcn_update_dpp(0, int(uint32_t(b)), CTRL, ROWMASK, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(uint32_t(b >> 32)), CTRL, ROWMASK, 0xF, true);
+  return __longlong_as_double((long long)(uint64_t(uint32_t(lo)) | (uint64_t(uint32_t(hi)) << 32)));
+}
+
+// Sum over the 64 lanes of a wavefront (every lane active), returned to every lane.  Quad
+// swaps, row rotations by 4 and 8 (row sums), then row broadcasts 15 and 31 carry rows 0-2 into
+// row 3; lane 63 ends with the total.
+__device__ __forceinline__ double wave_total(double x) {
+  x += dpp_f64<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
+  x += dpp_f64<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
+  x += dpp_f64<0x124, 0xF>(x);  // row_ror:4
+  x += dpp_f64<0x128, 0xF>(x);  // row_ror:8
+  x += dpp_f64<0x142, 0xA>(x);  // row_bcast:15 -> rows 1, 3
+  x += dpp_f64<0x143, 0xC>(x);  // row_bcast:31 -> rows 2, 3
+  const uint64_t b = uint64_t(__double_as_longlong(x));
+  return __longlong_as_double((long long)(uint64_t(lane_u32(uint32_t(b), 63)) | (uint64_t(lane_u32(uint32_t(b >> 32), 63)) << 32)));
+}
+
+// A record's stamp in tile coordinates: cell c (row c / S, column c % S of the S x S stamp) sits at
+// tile offset base + (c / S) * W + c % S; its CF value is cfp[c].
+struct Stamp {
+  const float2* cfp;
+  int S, S2, base;
+  float invS;
+};
+__device__ __forceinline__ Stamp stamp_of(uint64_t rec, const int* s_sup, const int64_t* s_off, const float2* cf,
+                                          int overs, int W, int x0, int y0) {
+  Stamp s;
+  const int plane = int((rec >> 32) & 0xFF);
+  s.S = uniform_i32(s_sup[plane]);
+  const int64_t coff = uniform_i64(s_off[plane]);
+  const int io = int((rec >> 40) & 0xFF), jo = int((rec >> 48) & 0xFF);
+  const int locx = int(rec & 0xFFFF), locy = int((rec >> 16) & 0xFFFF);
+  const int sup = (s.S - 1) / 2;
+  s.S2 = s.S * s.S;
+  s.cfp = cf + coff + int64_t(io * overs + jo) * s.S2;
+  s.base = (locy - sup - y0) * W + (locx - sup - x0);
+  s.invS = 1.f / float(s.S);
+  return s;
+}
+// Tile offset of cell c, or -1 past the stamp.  (c + 0.5) / S in FP32 is at least 0.5 / S from an
+// integer, far above its rounding error for any stamp that fits a tile, so the row is exact.
+__device__ __forceinline__ int stamp_cell(const Stamp& s, int c, int W) {
+  const int sy = int((float(c) + 0.5f) * s.invS);
+  return c < s.S2 ? sy * W + (c - sy * s.S) : -1;
+}
+
+// ---- cycle: grid pass B (LDS-tiled scatter) ------------------------------------------------------
+// One workgroup per bin item.  Each wavefront loads 64 records (placement, value) at once, then
+// stamps them one after the other with the record in scalar registers; lanes are stamp cells
+// (4 at a time per lane, CF loads issued together), and every polarisation of the record goes
+// into LDS (separate real / imaginary planes) with LDS float atomics.  The tile is then added
+// into the facet grid with global float atomics (non-zero cells only).
 template <bool CONJ>
 __global__ __launch_bounds__(256) void grid_pass_b(BatchParams p, const uint64_t* bplace, const float2* V,
-                                                   const BinItem* items, const uint32_t* recs, uint32_t* queue) {
-  extern __shared__ float tile[];  // [npg][W][W][2]
+                                                   const BinItem* items, const uint32_t* recs, uint32_t* queue,
+                                                   float* slots) {
+  extern __shared__ float tile[];  // [npg][2][W * W]
+  __shared__ int s_sup[256];
+  __shared__ int64_t s_off[256];
+  __shared__ uint32_t s_item;
   const int W = p.tile + 2 * p.halo;
+  const int WW = W * W;
   const int npg = p.nstokes < p.npol ? p.nstokes : p.npol;
+  const int nplanes = 2 * p.nw;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nwave = blockDim.x >> 6;
-  __shared__ uint32_t s_item;
+  for (int i = threadIdx.x; i < nplanes; i += blockDim.x) s_sup[i] = plane_support(p, i);
   while (true) {
     if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
     __syncthreads();
     const uint32_t it = s_item;
     if (it >= p.nitems_b) break;
     const BinItem bi = items[it];
-    for (int i = threadIdx.x; i < npg * W * W * 2; i += blockDim.x) tile[i] = 0.f;
+    const FacetDev& fd = p.facets[bi.facet];
+    const float2* cf = fd.cf;
+    for (int i = threadIdx.x; i < npg * 2 * WW; i += blockDim.x) tile[i] = 0.f;
+    for (int i = threadIdx.x; i < nplanes; i += blockDim.x) s_off[i] = fd.cf_offset[i];
     __syncthreads();
-    const int f = bi.facet;
-    const FacetDev& fd = p.facets[f];
     const int x0 = bi.tx * p.tile - p.halo, y0 = bi.ty * p.tile - p.halo;
-    // each wavefront takes 64 records at a time: lane j fetches record j's placement and values,
-    // then the wavefront stamps them one after the other (broadcast from lane j)
     for (uint32_t k0 = bi.begin + 64u * wave; k0 < bi.end; k0 += 64u * nwave) {
       const uint32_t k = k0 + lane;
       const bool ok = k < bi.end;
@@ -466,79 +639,145 @@ __global__ __launch_bounds__(256) void grid_pass_b(BatchParams p, const uint64_t
       for (int ip = 0; ip < 4; ++ip) my_v[ip] = (ok && ip < npg) ? V[size_t(my_t) * p.nstokes + ip] : make_float2(0.f, 0.f);
       const int cnt = int(min(64u, bi.end - k0));
       for (int j = 0; j < cnt; ++j) {
-        const uint64_t rec = __shfl(my_rec, j, 64);
-        const Unpacked u = unpack_place(rec);
-        const int S = plane_support(p, u.plane);
-        const int S2 = S * S, sup = (S - 1) / 2;
-        const float2* cfp = fd.cf + fd.cf_offset[u.plane] + int64_t(u.io * p.overs + u.jo) * S2;
-        float2 vv[4];
+        const Stamp s = stamp_of(lane_u64(my_rec, j), s_sup, s_off, cf, p.overs, W, x0, y0);
+        float vr[4], vi[4];
 #pragma unroll
-        for (int ip = 0; ip < 4; ++ip) vv[ip] = make_float2(__shfl(my_v[ip].x, j, 64), __shfl(my_v[ip].y, j, 64));
-        for (int c = lane; c < S2 * npg; c += 64) {
-          const int ip = c / S2;
-          const int kk = c - ip * S2;
-          const int sy = kk / S, sx = kk - sy * S;
-          float2 w = cfp[kk];
-          if (CONJ) w.y = -w.y;
-          const float2 val = ip == 0 ? vv[0] : (ip == 1 ? vv[1] : (ip == 2 ? vv[2] : vv[3]));
-          const float re = val.x * w.x - val.y * w.y;
-          const float im = val.x * w.y + val.y * w.x;
-          const int ty = u.locy - sup + sy - y0, tx = u.locx - sup + sx - x0;
-          float* cell = tile + ((size_t(ip) * W + ty) * W + tx) * 2;
-          atomicAdd(cell, re);
-          atomicAdd(cell + 1, im);
+        for (int ip = 0; ip < 4; ++ip) {
+          vr[ip] = lane_f32(my_v[ip].x, j);
+          vi[ip] = lane_f32(my_v[ip].y, j);
+        }
+        for (int c0 = 0; c0 < s.S2; c0 += 256) {
+          float2 w[4];
+          int off[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int c = c0 + 64 * r + lane;
+            off[r] = stamp_cell(s, c, W);
+            w[r] = off[r] >= 0 ? s.cfp[c] : make_float2(0.f, 0.f);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            if (off[r] < 0) continue;
+            const float wx = w[r].x, wy = CONJ ? -w[r].y : w[r].y;
+            float* t = tile + s.base + off[r];
+#pragma unroll
+            for (int ip = 0; ip < 4; ++ip) {
+              if (ip < npg) {
+                atomicAdd(t + (2 * ip) * WW, vr[ip] * wx - vi[ip] * wy);
+                atomicAdd(t + (2 * ip + 1) * WW, vr[ip] * wy + vi[ip] * wx);
+              }
+            }
+          }
         }
       }
     }
     __syncthreads();
-    float* grid = reinterpret_cast<float*>(fd.grid);
-    for (int i = threadIdx.x; i < npg * W * W; i += blockDim.x) {
-      const float re = tile[2 * i], im = tile[2 * i + 1];
-      if (re == 0.f && im == 0.f) continue;
-      const int ip = i / (W * W);
-      const int rem = i - ip * W * W;
-      const int gy = y0 + rem / W, gx = x0 + rem % W;
-      if (gx < 0 || gy < 0 || gx >= p.nx || gy >= p.ny) continue;
-      const size_t gi = ((size_t(bi.band) * p.npol + ip) * p.ny + gy) * p.nx + gx;
-      atomicAdd(&grid[2 * gi], re);
-      atomicAdd(&grid[2 * gi + 1], im);
+    if (slots) {
+      // the whole tile into this item's slot with plain stores (grid_slot_reduce adds the slots)
+      float4* dst = reinterpret_cast<float4*>(slots + size_t(it) * npg * 2 * WW);
+      const float4* src = reinterpret_cast<const float4*>(tile);
+      const int n4 = (npg * 2 * WW) / 4;
+      for (int i = threadIdx.x; i < n4; i += blockDim.x) dst[i] = src[i];
+      for (int i = 4 * n4 + threadIdx.x; i < npg * 2 * WW; i += blockDim.x)
+        slots[size_t(it) * npg * 2 * WW + i] = tile[i];
+    } else {
+      float* grid = reinterpret_cast<float*>(fd.grid);
+      for (int i = threadIdx.x; i < npg * WW; i += blockDim.x) {
+        const int ip = i / WW;
+        const int rem = i - ip * WW;
+        const float re = tile[2 * ip * WW + rem], im = tile[(2 * ip + 1) * WW + rem];
+        if (re == 0.f && im == 0.f) continue;
+        const int gy = y0 + rem / W, gx = x0 + rem % W;
+        if (gx < 0 || gy < 0 || gx >= p.nx || gy >= p.ny) continue;
+        const size_t gi = ((size_t(bi.band) * p.npol + ip) * p.ny + gy) * p.nx + gx;
+        atomicAdd(&grid[2 * gi], re);
+        atomicAdd(&grid[2 * gi + 1], im);
+      }
     }
     __syncthreads();
   }
 }
 
+// grid[f][band][pol][y][x] += the tile slots covering (y, x), one thread per grid cell: the
+// tiles ty with ty * T - h <= y < ty * T + T + h (at most two per axis, since h < T), each with
+// its consecutive work items, in item order.  No atomics; the same sum order on every run.
+__global__ __launch_bounds__(256) void grid_slot_reduce(BatchParams p, const float* slots, const uint32_t* tile_first,
+                                                        const uint32_t* tile_count) {
+  const int T = p.tile, h = p.halo;
+  const int W = T + 2 * h;
+  const int WW = W * W;
+  const int npg = p.nstokes < p.npol ? p.nstokes : p.npol;
+  const int64_t total = int64_t(p.nf) * p.nband * npg * p.ny * p.nx;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x) {
+    int64_t r = i;
+    const int x = int(r % p.nx); r /= p.nx;
+    const int y = int(r % p.ny); r /= p.ny;
+    const int ip = int(r % npg); r /= npg;
+    const int band = int(r % p.nband); r /= p.nband;
+    const int f = int(r);
+    float re = 0.f, im = 0.f;
+    const int ty0 = max(0, (y - h) / T), ty1 = min(p.ntiles_y - 1, (y + h) / T);
+    const int tx0 = max(0, (x - h) / T), tx1 = min(p.ntiles_x - 1, (x + h) / T);
+    for (int ty = ty0; ty <= ty1; ++ty) {
+      const int ly = y - (ty * T - h);
+      for (int tx = tx0; tx <= tx1; ++tx) {
+        const int lx = x - (tx * T - h);
+        const int64_t tid = ((int64_t(f) * p.nband + band) * p.ntiles_y + ty) * p.ntiles_x + tx;
+        const uint32_t first = tile_first[tid], n = tile_count[tid];
+        const float* t = slots + size_t(first) * npg * 2 * WW + size_t(2 * ip) * WW + size_t(ly) * W + lx;
+        for (uint32_t k = 0; k < n; ++k, t += size_t(npg) * 2 * WW) {
+          re += t[0];
+          im += t[WW];
+        }
+      }
+    }
+    if (re != 0.f || im != 0.f) {
+      float2* g = reinterpret_cast<float2*>(p.facets[f].grid) + ((size_t(band) * p.npol + ip) * p.ny + y) * p.nx + x;
+      const float2 o = *g;
+      *g = make_float2(o.x + re, o.y + im);
+    }
+  }
+}
+
 // ---- cycle: degrid pass A (LDS-staged gather) --------------------------------------------------
+// The model tile of a bin item is staged in LDS (separate real / imaginary planes); each record's
+// stamp is gathered by one wavefront: FP32 products (grid x CF) summed in FP64, per lane and then
+// across the wavefront (degridder.h:202-212 sums the FP32 products in double).
 __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64_t* bplace, float2* M,
                                                      const BinItem* items, const uint32_t* recs, uint32_t* queue) {
-  extern __shared__ float tile[];  // [nstokes][W][W][2]
+  extern __shared__ float tile[];  // [nstokes][2][W * W]
+  __shared__ int s_sup[256];
+  __shared__ int64_t s_off[256];
+  __shared__ uint32_t s_item;
   const int W = p.tile + 2 * p.halo;
+  const int WW = W * W;
+  const int ns = p.nstokes;
+  const int nplanes = 2 * p.nw;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nwave = blockDim.x >> 6;
-  __shared__ uint32_t s_item;
+  for (int i = threadIdx.x; i < nplanes; i += blockDim.x) s_sup[i] = plane_support(p, i);
   while (true) {
     if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
     __syncthreads();
     const uint32_t it = s_item;
     if (it >= p.nitems_b) break;
     const BinItem bi = items[it];
-    const int f = bi.facet;
-    const FacetDev& fd = p.facets[f];
+    const FacetDev& fd = p.facets[bi.facet];
+    const float2* cf = fd.cf;
     const int x0 = bi.tx * p.tile - p.halo, y0 = bi.ty * p.tile - p.halo;
-    const float* model = reinterpret_cast<const float*>(fd.model);
-    for (int i = threadIdx.x; i < p.nstokes * W * W; i += blockDim.x) {
-      const int ip = i / (W * W);
-      const int rem = i - ip * W * W;
+    const float2* model = reinterpret_cast<const float2*>(fd.model);
+    for (int i = threadIdx.x; i < ns * WW; i += blockDim.x) {
+      const int ip = i / WW;
+      const int rem = i - ip * WW;
       const int gy = y0 + rem / W, gx = x0 + rem % W;
-      float re = 0.f, im = 0.f;
-      if (gx >= 0 && gy >= 0 && gx < p.nx && gy < p.ny) {
-        const size_t gi = ((size_t(bi.band) * p.npol + ip) * p.ny + gy) * p.nx + gx;
-        re = model[2 * gi];
-        im = model[2 * gi + 1];
-      }
-      tile[2 * i] = re;
-      tile[2 * i + 1] = im;
+      float2 m = make_float2(0.f, 0.f);
+      if (gx >= 0 && gy >= 0 && gx < p.nx && gy < p.ny)
+        m = model[((size_t(bi.band) * p.npol + ip) * p.ny + gy) * p.nx + gx];
+      tile[2 * ip * WW + rem] = m.x;
+      tile[(2 * ip + 1) * WW + rem] = m.y;
     }
+    for (int i = threadIdx.x; i < nplanes; i += blockDim.x) s_off[i] = fd.cf_offset[i];
     __syncthreads();
     for (uint32_t k0 = bi.begin + 64u * wave; k0 < bi.end; k0 += 64u * nwave) {
       const uint32_t k = k0 + lane;
@@ -548,30 +787,35 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
       float2 res[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
       const int cnt = int(min(64u, bi.end - k0));
       for (int j = 0; j < cnt; ++j) {
-        const uint64_t rec = __shfl(my_rec, j, 64);
-        const Unpacked u = unpack_place(rec);
-        const int S = plane_support(p, u.plane);
-        const int S2 = S * S, sup = (S - 1) / 2;
-        const float2* cfp = fd.cf + fd.cf_offset[u.plane] + int64_t(u.io * p.overs + u.jo) * S2;
+        const Stamp s = stamp_of(lane_u64(my_rec, j), s_sup, s_off, cf, p.overs, W, x0, y0);
         double sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
-        for (int c = lane; c < S2 * p.nstokes; c += 64) {
-          const int ip = c / S2;
-          const int kk = c - ip * S2;
-          const int sy = kk / S, sx = kk - sy * S;
-          const int ty = u.locy - sup + sy - y0, tx = u.locx - sup + sx - x0;
-          const float* cell = tile + ((size_t(ip) * W + ty) * W + tx) * 2;
-          const float2 w = cfp[kk];
-          const float pr = cell[0] * w.x - cell[1] * w.y;
-          const float pi = cell[0] * w.y + cell[1] * w.x;
-          if (ip == 0) { sr[0] += pr; si[0] += pi; }
-          else if (ip == 1) { sr[1] += pr; si[1] += pi; }
-          else if (ip == 2) { sr[2] += pr; si[2] += pi; }
-          else { sr[3] += pr; si[3] += pi; }
+        for (int c0 = 0; c0 < s.S2; c0 += 256) {
+          float2 w[4];
+          int off[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int c = c0 + 64 * r + lane;
+            off[r] = stamp_cell(s, c, W);
+            w[r] = off[r] >= 0 ? s.cfp[c] : make_float2(0.f, 0.f);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            if (off[r] < 0) continue;
+            const float* t = tile + s.base + off[r];
+#pragma unroll
+            for (int ip = 0; ip < 4; ++ip) {
+              if (ip < ns) {
+                const float mr = t[(2 * ip) * WW], mi = t[(2 * ip + 1) * WW];
+                sr[ip] += double(mr * w[r].x - mi * w[r].y);
+                si[ip] += double(mr * w[r].y + mi * w[r].x);
+              }
+            }
+          }
         }
 #pragma unroll
         for (int ip = 0; ip < 4; ++ip) {
-          if (ip < p.nstokes) {
-            const double a = wave_sum(sr[ip]), b = wave_sum(si[ip]);
+          if (ip < ns) {
+            const double a = wave_total(sr[ip]), b = wave_total(si[ip]);
             if (lane == j) res[ip] = make_float2(float(a), float(b));
           }
         }
@@ -579,7 +823,7 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
       if (ok) {
 #pragma unroll
         for (int ip = 0; ip < 4; ++ip)
-          if (ip < p.nstokes) M[size_t(my_t) * p.nstokes + ip] = res[ip];
+          if (ip < ns) M[size_t(my_t) * ns + ip] = res[ip];
       }
     }
     __syncthreads();
@@ -637,7 +881,7 @@ __global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v
       if (JMODE == 2) {
         const int64_t mid = br.rows[br.nr / 2];
         if (mid >= 0 && mid < v.nrow) {
-          JonesPair jj = jones_at(jp, mid, (br.ch0 + br.ch1) / 2, v.uvw + 3 * mid, p.wl_mean);
+          JonesPair jj = jones_at<true>(jp, mid, (br.ch0 + br.ch1) / 2, v.uvw + 3 * mid, p.wl_mean);
           cv = mul(mul(jj.J0, cv), herm(jj.J1));
         }
       }
@@ -652,7 +896,7 @@ __global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v
       const double angle = 2. * kPi * (uvw[0] * fd.l0 + uvw[1] * fd.m0 + uvw[2] * fd.n0) / kC;
       const cd corr = phasor_prec<PREC>(fr, angle) * dec;
       if (JMODE == 1) {
-        JonesPair jj = jones_at(jp, row, c, uvw, p.wl_mean);
+        JonesPair jj = jones_at<true>(jp, row, c, uvw, p.wl_mean);
         cv = mul(mul(jj.J0, cv), herm(jj.J1));
       }
 #pragma unroll
@@ -735,7 +979,7 @@ __global__ __launch_bounds__(256) void degrid_adj_b(BatchParams p, const VisP v,
           }
           const cd cc = conj(phasor_prec<PREC>(v.freqs[c], angle)) * dec;
           if (JMODE == 1) {
-            JonesPair jj = jones_at(jp, row, c, uvw, p.wl_mean);
+            JonesPair jj = jones_at<true>(jp, row, c, uvw, p.wl_mean);
             m = mul(mul(herm(jj.J0), m), jj.J1);
           }
 #pragma unroll
@@ -743,7 +987,7 @@ __global__ __launch_bounds__(256) void degrid_adj_b(BatchParams p, const VisP v,
         }
       }
       if (JMODE == 2 && mid_ok) {
-        JonesPair jj = jones_at(jp, mid, (br.ch0 + br.ch1) / 2, v.uvw + 3 * mid, p.wl_mean);
+        JonesPair jj = jones_at<true>(jp, mid, (br.ch0 + br.ch1) / 2, v.uvw + 3 * mid, p.wl_mean);
         M2 m{{acc[0], acc[1], acc[2], acc[3]}};
         m = mul(mul(herm(jj.J0), m), jj.J1);
 #pragma unroll
@@ -833,7 +1077,10 @@ static void launch_a_p(const BatchParams& p, const VisP& v, const uint64_t* plac
 template <int READ>
 static void launch_a_j(const BatchParams& p, const VisP& v, const uint64_t* place, float2* V, uint32_t* q,
                        size_t lds, unsigned nwg, hipStream_t st) {
-  if (p.jmode == 1) launch_a_p<READ, 1>(p, v, place, V, q, lds, nwg, st);
+  // per-visibility Jones: run-accumulated (1), or per visibility with ScaleJones (3), whose Jones
+  // also depends on the uvw of the visibility that refreshed it
+  if (p.jmode == 1 && p.jp.scale) launch_a_p<READ, 3>(p, v, place, V, q, lds, nwg, st);
+  else if (p.jmode == 1) launch_a_p<READ, 1>(p, v, place, V, q, lds, nwg, st);
   else if (p.jmode == 2) launch_a_p<READ, 2>(p, v, place, V, q, lds, nwg, st);
   else launch_a_p<READ, 0>(p, v, place, V, q, lds, nwg, st);
 }
@@ -847,12 +1094,44 @@ int batched_grid_a(const BatchParams& p, const VisP& v, const uint64_t* place, f
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
+// Persistent grid size: `nwg` when given, else as many workgroups as fit on the device at once.
+template <typename K>
+static unsigned persistent_wgs(K kernel, size_t lds, unsigned nwg) {
+  if (nwg) return nwg;
+  int dev = 0, ncu = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 1024;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, lds) != hipSuccess || per < 1) per = 1;
+  return unsigned(ncu * per);
+}
+
+// Persistent grid sizes of the tile passes (grid B, degrid A, adjoint of degrid A), computed once
+// per plan so that the launches make no device queries (they may run under stream capture).
+int batched_tile_wgs(const BatchParams& pg, const BatchParams& pd, unsigned out[3]) {
+  const int Wg = pg.tile + 2 * pg.halo, Wd = pd.tile + 2 * pd.halo;
+  const int npg = pg.nstokes < pg.npol ? pg.nstokes : pg.npol;
+  out[0] = persistent_wgs(grid_pass_b<false>, size_t(npg) * Wg * Wg * 2 * sizeof(float), 0);
+  out[1] = persistent_wgs(degrid_pass_a, size_t(pd.nstokes) * Wd * Wd * 2 * sizeof(float), 0);
+  out[2] = persistent_wgs(grid_pass_b<true>, size_t(pd.nstokes) * Wd * Wd * 2 * sizeof(float), 0);
+  return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
+}
+
 int batched_grid_b(const BatchParams& p, const uint64_t* place, const float2* V, const BinItem* items,
-                   const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st) {
+                   const uint32_t* recs, uint32_t* queue, unsigned nwg, float* slots, const uint32_t* tile_first,
+                   const uint32_t* tile_count, hipStream_t st) {
   const int W = p.tile + 2 * p.halo;
   const int npg = p.nstokes < p.npol ? p.nstokes : p.npol;
   size_t lds = size_t(npg) * W * W * 2 * sizeof(float);
-  hipLaunchKernelGGL(grid_pass_b<false>, dim3(nwg), dim3(256), lds, st, p, place, V, items, recs, queue);
+  nwg = persistent_wgs(grid_pass_b<false>, lds, nwg);
+  hipLaunchKernelGGL(grid_pass_b<false>, dim3(nwg), dim3(256), lds, st, p, place, V, items, recs, queue, slots);
+  if (hipGetLastError() != hipSuccess) return DDF_EHIP;
+  if (slots) {
+    const int64_t total = int64_t(p.nf) * p.nband * npg * p.ny * p.nx;
+    int64_t nb = (total + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(grid_slot_reduce, dim3(unsigned(nb)), dim3(256), 0, st, p, slots, tile_first, tile_count);
+  }
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
@@ -863,7 +1142,9 @@ int batched_degrid_a_adjoint(const BatchParams& p, const uint64_t* place, const 
   size_t lds = size_t(p.nstokes) * W * W * 2 * sizeof(float);
   BatchParams q = p;
   q.npol = p.npol;
-  hipLaunchKernelGGL(grid_pass_b<true>, dim3(nwg), dim3(256), lds, st, q, place, Mbar, items, recs, queue);
+  nwg = persistent_wgs(grid_pass_b<true>, lds, nwg);
+  hipLaunchKernelGGL(grid_pass_b<true>, dim3(nwg), dim3(256), lds, st, q, place, Mbar, items, recs, queue,
+                     static_cast<float*>(nullptr));
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
@@ -871,6 +1152,7 @@ int batched_degrid_a(const BatchParams& p, const uint64_t* place, float2* M, con
                      const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st) {
   const int W = p.tile + 2 * p.halo;
   size_t lds = size_t(p.nstokes) * W * W * 2 * sizeof(float);
+  nwg = persistent_wgs(degrid_pass_a, lds, nwg);
   hipLaunchKernelGGL(degrid_pass_a, dim3(nwg), dim3(256), lds, st, p, place, M, items, recs, queue);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
@@ -945,13 +1227,13 @@ int batched_gather_place(const uint64_t* place, const uint32_t* recs, int64_t n,
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
-int batched_transpose_jones(const float2* src, float2* dst, const int* dims, hipStream_t st) {
+int batched_prepare_jones(const BatchParams& p, const float2* src, double2* dst, int dd, hipStream_t st) {
+  const int* dims = dd ? p.jp.kdims : p.jp.bdims;
   int64_t total = int64_t(dims[0]) * dims[1] * dims[2] * dims[3];
   int64_t nb = (total + 255) / 256;
   if (nb > 65536) nb = 65536;
   if (nb < 1) nb = 1;
-  hipLaunchKernelGGL(plan_transpose_jones, dim3(unsigned(nb)), dim3(256), 0, st, src, dst, dims[0], dims[1], dims[2],
-                     dims[3]);
+  hipLaunchKernelGGL(plan_prepare_jones, dim3(unsigned(nb)), dim3(256), 0, st, src, dst, p.jp, dd);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 

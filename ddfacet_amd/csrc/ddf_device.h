@@ -72,7 +72,6 @@ __device__ __forceinline__ cd phasor(double phase) {
 // ---- Jones tables (JonesServer.cc) -----------------------------------------------------------
 struct JonesP {
   int enabled, mode, interp;
-  int tl;  // table layout: 0 the reference [t][dir][ant][chan][4], 1 facet-fastest [t][ant][chan][dir][4]
   const float2* kms;
   int kdims[4];
   const int32_t *ktime, *kchan;
@@ -88,19 +87,23 @@ struct JonesP {
   double calib_error;
   double* sum_jones;
   double* sum_jones_chan;
+  // prepared tables of the batched plan (facet-fastest [t][ant][chan][dir][4], FP64): the DD
+  // table with interpolation and NormJones amplitude / phase applied, the beam table with
+  // interpolation applied; null when the raw tables above are read
+  const double2* pk;
+  const double2* pb;
 };
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // GiveJones (JonesServer.cc:54-95); indices clamped to the table so a bad map cannot fault.
 __device__ inline M2 give_jones(const float2* tab, const int* dims, int it, int ant, int dir, int ich,
-                                int interp, const float* coefs, int ncoefs, int tl) {
+                                int interp, const float* coefs, int ncoefs) {
   it = clampi(it, 0, dims[0] - 1);
   dir = clampi(dir, 0, dims[1] - 1);
   ant = clampi(ant, 0, dims[2] - 1);
   ich = clampi(ich, 0, dims[3] - 1);
-  size_t off = tl ? ((size_t(it) * dims[2] + ant) * size_t(dims[3]) + ich) * size_t(dims[1]) + dir
-                  : ((size_t(it) * dims[1] + dir) * size_t(dims[2]) + ant) * size_t(dims[3]) + ich;
+  size_t off = ((size_t(it) * dims[1] + dir) * size_t(dims[2]) + ant) * size_t(dims[3]) + ich;
   const float2* p = tab + off * 4;
   M2 out;
   if (interp == 0) {
@@ -130,6 +133,18 @@ __device__ inline M2 give_jones(const float2* tab, const int* dims, int it, int 
   return out;
 }
 
+// ScaleJones part of NormJones (JonesServer.cc:44-51): amplitudes pulled towards 1 by a factor
+// that depends on the baseline length of the visibility.
+__device__ inline void scale_jones(const JonesP& jp, M2& J, const double* uvw, double wl_mean) {
+  double R2 = (uvw[0] * uvw[0] + uvw[1] * uvw[1]) / (wl_mean * wl_mean);
+  double al = exp(-2. * kPi * jp.calib_error * jp.calib_error * R2);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double a = cabs(J.a[q]);
+    if (a != 0.) J.a[q] = J.a[q] * ((1. - al) / a + al);
+  }
+}
+
 // NormJones (JonesServer.cc:25-52), applied to the DD (killMS) Jones only.
 __device__ inline void norm_jones(const JonesP& jp, M2& J, const double* uvw, double wl_mean) {
   if (!jp.apply_amp) {
@@ -143,15 +158,7 @@ __device__ inline void norm_jones(const JonesP& jp, M2& J, const double* uvw, do
 #pragma unroll
     for (int q = 0; q < 4; ++q) J.a[q] = cd{cabs(J.a[q]), 0.};
   }
-  if (jp.scale) {
-    double R2 = (uvw[0] * uvw[0] + uvw[1] * uvw[1]) / (wl_mean * wl_mean);
-    double al = exp(-2. * kPi * jp.calib_error * jp.calib_error * R2);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      double a = cabs(J.a[q]);
-      if (a != 0.) J.a[q] = J.a[q] * ((1. - al) / a + al);
-    }
-  }
+  if (jp.scale) scale_jones(jp, J, uvw, wl_mean);
 }
 
 struct JonesPair {
@@ -161,19 +168,59 @@ struct JonesPair {
 
 // updateJones (JonesServer.cc:193-298) evaluated afresh for (row, chan): J = JkMS * JBeam per
 // antenna, BB = ((|J0_00||J1_00| + |J0_11||J1_11|)/2)^2.  `uvw` feeds ScaleJones only.
+// Entry of a prepared table (JonesP::pk / pb), indices clamped like give_jones.
+__device__ __forceinline__ M2 prepared_jones(const double2* tab, const int* dims, int it, int ant, int dir, int ich) {
+  it = clampi(it, 0, dims[0] - 1);
+  dir = clampi(dir, 0, dims[1] - 1);
+  ant = clampi(ant, 0, dims[2] - 1);
+  ich = clampi(ich, 0, dims[3] - 1);
+  const double2* p = tab + (((size_t(it) * dims[2] + ant) * size_t(dims[3]) + ich) * size_t(dims[1]) + dir) * 4;
+  M2 out;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out.a[q] = cd{p[q].x, p[q].y};
+  return out;
+}
+
+// PREP: read the prepared tables (JonesP::pk / pb, the batched plan), else the raw tables.  Both
+// give the same values; the prepared path has the per-entry work done once.
+template <bool PREP = false>
 __device__ inline JonesPair jones_at(const JonesP& jp, int64_t row, int chan, const double* uvw,
                                      double wl_mean) {
   int a0 = jp.a0[row], a1 = jp.a1[row];
+  if (PREP) {
+    JonesPair r;
+    if (jp.pk) {
+      const int t = jp.ktime[row], c = jp.kchan[chan];
+      r.J0 = prepared_jones(jp.pk, jp.kdims, t, a0, jp.kdir, c);
+      r.J1 = prepared_jones(jp.pk, jp.kdims, t, a1, jp.kdir, c);
+      if (jp.scale) {
+        scale_jones(jp, r.J0, uvw, wl_mean);
+        scale_jones(jp, r.J1, uvw, wl_mean);
+      }
+      if (jp.pb) {
+        const int tb = jp.btime[row], cb = jp.bchan[chan];
+        r.J0 = mul(r.J0, prepared_jones(jp.pb, jp.bdims, tb, a0, jp.bdir, cb));
+        r.J1 = mul(r.J1, prepared_jones(jp.pb, jp.bdims, tb, a1, jp.bdir, cb));
+      }
+    } else {
+      const int tb = jp.btime[row], cb = jp.bchan[chan];
+      r.J0 = prepared_jones(jp.pb, jp.bdims, tb, a0, jp.bdir, cb);
+      r.J1 = prepared_jones(jp.pb, jp.bdims, tb, a1, jp.bdir, cb);
+    }
+    const double bb = (cabs(r.J0.a[0]) * cabs(r.J1.a[0]) + cabs(r.J0.a[3]) * cabs(r.J1.a[3])) / 2.;
+    r.BB = bb * bb;
+    return r;
+  }
   M2 J0 = unity(), J1 = unity();
   if (jp.beam) {
     int t = jp.btime[row], c = jp.bchan[chan];
-    J0 = give_jones(jp.beam, jp.bdims, t, a0, jp.bdir, c, jp.interp, jp.coefs, jp.ncoefs, jp.tl);
-    J1 = give_jones(jp.beam, jp.bdims, t, a1, jp.bdir, c, jp.interp, jp.coefs, jp.ncoefs, jp.tl);
+    J0 = give_jones(jp.beam, jp.bdims, t, a0, jp.bdir, c, jp.interp, jp.coefs, jp.ncoefs);
+    J1 = give_jones(jp.beam, jp.bdims, t, a1, jp.bdir, c, jp.interp, jp.coefs, jp.ncoefs);
   }
   if (jp.kms) {
     int t = jp.ktime[row], c = jp.kchan[chan];
-    M2 K0 = give_jones(jp.kms, jp.kdims, t, a0, jp.kdir, c, jp.interp, jp.coefs, jp.ncoefs, jp.tl);
-    M2 K1 = give_jones(jp.kms, jp.kdims, t, a1, jp.kdir, c, jp.interp, jp.coefs, jp.ncoefs, jp.tl);
+    M2 K0 = give_jones(jp.kms, jp.kdims, t, a0, jp.kdir, c, jp.interp, jp.coefs, jp.ncoefs);
+    M2 K1 = give_jones(jp.kms, jp.kdims, t, a1, jp.kdir, c, jp.interp, jp.coefs, jp.ncoefs);
     norm_jones(jp, K0, uvw, wl_mean);
     norm_jones(jp, K1, uvw, wl_mean);
     J0 = mul(K0, J0);
@@ -192,6 +239,7 @@ __device__ inline JonesPair jones_at(const JonesP& jp, int64_t row, int chan, co
 // the uvw of the visibility that triggered the refresh (which only the ScaleJones term reads).
 // Beam and DD parts are refreshed together (the reference refreshes them separately; the two
 // differ only with ScaleJones on and a beam-only key change).
+template <bool PREP = false>
 struct JonesCache {
   int a0, a1, kt, kc, bt, bc;
   JonesPair cur;
@@ -202,7 +250,7 @@ struct JonesCache {
     const int k_t = jp.kms ? jp.ktime[row] : -1, k_c = jp.kms ? jp.kchan[chan] : -1;
     const int b_t = jp.beam ? jp.btime[row] : -1, b_c = jp.beam ? jp.bchan[chan] : -1;
     if (r0 != a0 || r1 != a1 || k_t != kt || k_c != kc || b_t != bt || b_c != bc) {
-      cur = jones_at(jp, row, chan, uvw, wl_mean);
+      cur = jones_at<PREP>(jp, row, chan, uvw, wl_mean);
       a0 = r0; a1 = r1; kt = k_t; kc = k_c; bt = b_t; bc = b_c;
     }
     return cur;

@@ -21,6 +21,7 @@ struct VisP {
   const int64_t* bda_start;
   int64_t nblocks, bda_len;
   const uint8_t* sparse;
+  int equi;  // channels equidistant (LOptimisation[1])
 };
 
 struct BlockRec {
@@ -67,6 +68,41 @@ __device__ __forceinline__ cd phasor_prec(double freq, double angle) {
     return phasor(freq * angle);
   }
 }
+
+// Phasors exp(i freq_c angle) along the channels of one row, for a lane that walks them in
+// increasing order.  PREC 0 with equidistant channels: FP64 phase base, FP32 sincos at an anchor
+// (the first channel walked, a channel after a gap, every multiple of 16), and in between the
+// phasor advanced in FP32 by the per-row step exp(i (f1 - f0) angle).  Other precisions, or
+// non-equidistant channels: every phasor formed directly by phasor_prec.
+template <int PREC>
+struct ChanPhasor {
+  double angle;
+  int last;
+  float cr, ci, sr, si;
+  __device__ __forceinline__ void row(double a) {
+    angle = a;
+    last = -2;
+  }
+  __device__ __forceinline__ cd at(const VisP& v, int c) {
+    if (PREC != 0 || !v.equi) return phasor_prec<PREC>(v.freqs[c], angle);
+    if (c == last + 1 && (c & 15) != 0) {
+      const float r = cr * sr - ci * si, i = cr * si + ci * sr;
+      cr = r;
+      ci = i;
+    } else {
+      const cd a = phasor(v.freqs[c] * angle);
+      cr = float(a.re);
+      ci = float(a.im);
+      if (last == -2) {
+        const cd st = phasor((v.freqs[1] - v.freqs[0]) * angle);
+        sr = float(st.re);
+        si = float(st.im);
+      }
+    }
+    last = c;
+    return cd{cr, ci};
+  }
+};
 
 // One facet as the batched kernels see it.
 struct FacetDev {
@@ -126,8 +162,11 @@ int batched_vlist(const VisP& v, const int64_t* vis_start, uint64_t* vrc, uint32
 int batched_reset(uint32_t* counters, int n, hipStream_t st);
 int batched_grid_a(const BatchParams& p, const VisP& v, const uint64_t* place, float2* V, uint32_t* queue,
                    unsigned nwg, hipStream_t st);
+// slots: null -> tiles added into the grids with global atomics; else one tile slot per work item
+// ([nitems][npg][2][W*W]) and grid_slot_reduce (tile_first / tile_count: the items of each tile)
 int batched_grid_b(const BatchParams& p, const uint64_t* place, const float2* V, const BinItem* items,
-                   const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st);
+                   const uint32_t* recs, uint32_t* queue, unsigned nwg, float* slots, const uint32_t* tile_first,
+                   const uint32_t* tile_count, hipStream_t st);
 int batched_degrid_a(const BatchParams& p, const uint64_t* place, float2* M, const BinItem* items,
                      const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st);
 int batched_degrid_b(const BatchParams& p, const VisP& v, const float2* M, const uint64_t* vrc, const uint32_t* vblk,
@@ -138,7 +177,8 @@ int batched_degrid_b_adjoint(const BatchParams& p, const VisP& v, const uint64_t
 int batched_degrid_a_adjoint(const BatchParams& p, const uint64_t* place, const float2* Mbar, const BinItem* items,
                              const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st);
 
+int batched_tile_wgs(const BatchParams& pg, const BatchParams& pd, unsigned out[3]);
 int batched_gather_place(const uint64_t* place, const uint32_t* recs, int64_t n, uint64_t* bplace, hipStream_t st);
-int batched_transpose_jones(const float2* src, float2* dst, const int* dims, hipStream_t st);
+int batched_prepare_jones(const BatchParams& p, const float2* src, double2* dst, int dd, hipStream_t st);
 
 }  // namespace ddf

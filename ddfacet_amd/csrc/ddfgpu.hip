@@ -914,6 +914,9 @@ struct ddf_plan {
   uint32_t* recs_g = nullptr;
   uint32_t* recs_d = nullptr;
   uint64_t* bplace_g = nullptr;  // placement records in bin order
+  float* slots_g = nullptr;      // one LDS tile image per grid work item (null: atomic flush)
+  uint32_t* tile_first_g = nullptr;
+  uint32_t* tile_count_g = nullptr;
   uint64_t* bplace_d = nullptr;
   BinItem* items_g = nullptr;
   BinItem* items_d = nullptr;
@@ -923,11 +926,26 @@ struct ddf_plan {
   uint32_t* vblk = nullptr;
   int64_t nvl = 0;
   uint32_t* queues = nullptr;
+  const float2* jsrc[2] = {nullptr, nullptr};  // caller's DD / beam Jones tables
+  double2* jprep[2] = {nullptr, nullptr};      // prepared copies (JonesP::pk / pb)
   unsigned nwg = 0;
+  unsigned nwg_tiles[3] = {0, 0, 0};  // tile passes (grid B, degrid A, adjoint): as many as fit
   int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   std::vector<void*> allocs;
+  // per-pass timing (ddf_plan_timing): events around each pass on the launch stream
+  int timing = 0;
+  hipEvent_t ev[DDF_PLAN_NPASS][2] = {};
+  int ev_set[DDF_PLAN_NPASS] = {};
   ~ddf_plan() {
     for (void* p : allocs) (void)hipFree(p);
+    for (auto& e : ev)
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+  }
+  void mark(int pass, int end, hipStream_t st) {
+    if (!timing) return;
+    (void)hipEventRecord(ev[pass][end], st);
+    if (end) ev_set[pass] = 1;
   }
   template <typename T>
   T* alloc(size_t n) {
@@ -942,6 +960,12 @@ struct ddf_plan {
 };
 
 namespace {
+
+int prepare_jones(ddf_plan* P, hipStream_t st) {
+  for (int k = 0; k < 2; ++k)
+    if (P->jprep[k] && batched_prepare_jones(P->pg, P->jsrc[k], P->jprep[k], k == 0, st)) return DDF_EHIP;
+  return DDF_OK;
+}
 
 int fill_vis(VisP& out, const ddf_vis* v, const int64_t* starts) {
   std::memset(&out, 0, sizeof(out));
@@ -959,6 +983,7 @@ int fill_vis(VisP& out, const ddf_vis* v, const int64_t* starts) {
   out.nblocks = v->nblocks;
   out.bda_len = v->bda_len;
   out.sparse = v->sparsification;
+  out.equi = v->chan_equidistant && v->nchan > 1;
   return DDF_OK;
 }
 
@@ -1013,7 +1038,8 @@ void fill_batch(BatchParams& p, const ddf_grid_desc* g, const ddf_facet* f0, con
 
 // Count -> scan -> fill of the valid records, then the work items (largest first).
 int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place, int chunk, uint32_t** recs_out,
-               BinItem** items_out, uint32_t* nitems, int64_t* nrec, hipStream_t st) {
+               BinItem** items_out, uint32_t* nitems, int64_t* nrec, hipStream_t st, uint32_t** tile_first_out = nullptr,
+               uint32_t** tile_count_out = nullptr) {
   const int64_t nsub = 2 * int64_t(p.nw);  // CF-plane sub-bins of each tile
   const int64_t ntile_all = int64_t(p.nf) * p.nband * p.ntiles_y * p.ntiles_x;
   const int64_t nbins = ntile_all * nsub;
@@ -1030,7 +1056,9 @@ int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place
   std::vector<int64_t> hs(static_cast<size_t>(nbins));
   int64_t tot = 0;
   std::vector<BinItem> items;
+  std::vector<uint32_t> tfirst(static_cast<size_t>(ntile_all)), tcount(static_cast<size_t>(ntile_all));
   for (int64_t i = 0; i < ntile_all; ++i) {
+    tfirst[size_t(i)] = uint32_t(items.size());
     int64_t c = 0;
     for (int64_t q = 0; q < nsub; ++q) {
       hs[size_t(i * nsub + q)] = tot + c;
@@ -1053,6 +1081,7 @@ int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place
         items.push_back(it);
       }
     }
+    tcount[size_t(i)] = uint32_t(items.size()) - tfirst[size_t(i)];
     tot += c;
   }
   if (tot >= (int64_t(1) << 32)) return fail(DDF_EINVAL, "too many (block, facet) records for one plan");
@@ -1064,6 +1093,15 @@ int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place
   if (!recs || !d_items) return fail(DDF_ENOMEM, "plan allocation failed");
   if (!items.empty())
     HIP_TRY(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(BinItem), hipMemcpyHostToDevice, st));
+  if (tile_first_out) {
+    uint32_t* tf = P->alloc<uint32_t>(size_t(ntile_all));
+    uint32_t* tc = P->alloc<uint32_t>(size_t(ntile_all));
+    if (!tf || !tc) return fail(DDF_ENOMEM, "plan allocation failed");
+    HIP_TRY(hipMemcpyAsync(tf, tfirst.data(), size_t(ntile_all) * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(tc, tcount.data(), size_t(ntile_all) * 4, hipMemcpyHostToDevice, st));
+    *tile_first_out = tf;
+    *tile_count_out = tc;
+  }
   if (batched_fill(p, v.nblocks, place, bstart, fill, recs, st)) return fail(DDF_EHIP, "plan_fill launch failed");
   HIP_TRY(hipStreamSynchronize(st));
   *recs_out = recs;
@@ -1135,29 +1173,40 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   fill_batch(P->pg, g, &facets[0], smg, fm, wl);
   fill_batch(P->pd, g, &facets[0], smd, fm, wl);
   P->pg.dopsf = g->dopsf;
-  // facet-fastest copies of the Jones tables: the lanes of a wavefront are facets, so their
-  // Jones reads become contiguous
+  // prepared Jones tables (FP64, facet-fastest), refreshed from the caller's tables at every
+  // grid / degrid call so that new Jones values need no new plan
   for (int k = 0; k < 2; ++k) {
     const ddf_jones& j = facets[0].jones;
     const float* src = k ? j.beam : j.kms;
     const int32_t* dims = k ? j.beam_dims : j.kms_dims;
     if (!j.enabled || !src) continue;
     const size_t n = size_t(dims[0]) * dims[1] * dims[2] * dims[3] * 4;
-    float2* dst = P->alloc<float2>(n);
+    double2* dst = P->alloc<double2>(n);
     if (!dst) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
-    if (batched_transpose_jones(reinterpret_cast<const float2*>(src), dst, dims, st))
-      return bail(fail(DDF_EHIP, "Jones transpose failed"));
+    P->jsrc[k] = reinterpret_cast<const float2*>(src);
+    P->jprep[k] = dst;
     for (BatchParams* bp : {&P->pg, &P->pd}) {
-      if (k) bp->jp.beam = dst;
-      else bp->jp.kms = dst;
-      bp->jp.tl = 1;
+      if (k) bp->jp.pb = dst;
+      else bp->jp.pk = dst;
     }
   }
+  if (prepare_jones(P, st)) return bail(fail(DDF_EHIP, "Jones preparation failed"));
   ddf_plan_opts o{};
   if (opts) o = *opts;
   const int tile = o.tile > 0 ? o.tile : 32;
   const int chunk = o.chunk > 0 ? o.chunk : 2048;
   const int halo = (maxsup - 1) / 2;
+  {
+    // the tile passes hold one (T + 2 h)^2 complex64 image per polarisation in LDS
+    int dev = 0, lds_max = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      return bail(fail(DDF_EHIP, "device query failed"));
+    const int W = tile + 2 * halo;
+    const int np = std::max(std::min(smg.nstokes, g->npol), smd.nstokes);
+    if (size_t(np) * W * W * 8 + 4096 > size_t(lds_max))
+      return bail(fail(DDF_EINVAL, "uv tile too large for LDS at this support and polarisation count"));
+  }
   int G = o.group;
   if (G <= 0) {
     int ng = (nfacets + 31) / 32;
@@ -1232,7 +1281,8 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   // bins
   uint32_t ni = 0;
   int64_t nr = 0;
-  rc = build_bins(P, P->pg, P->vg, P->place_g, chunk, &P->recs_g, &P->items_g, &ni, &nr, st);
+  rc = build_bins(P, P->pg, P->vg, P->place_g, chunk, &P->recs_g, &P->items_g, &ni, &nr, st, &P->tile_first_g,
+                  &P->tile_count_g);
   if (rc) return bail(rc);
   P->pg.nitems_b = ni;
   P->stats[0] = nr;
@@ -1240,6 +1290,17 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   P->bplace_g = P->alloc<uint64_t>(size_t(nr));
   if (!P->bplace_g || batched_gather_place(P->place_g, P->recs_g, nr, P->bplace_g, st))
     return bail(fail(DDF_ENOMEM, "plan allocation failed"));
+  {
+    // tile slots of the gridding items (plain stores + a deterministic reduction instead of
+    // global atomics); without the memory for them the tiles are flushed with atomics
+    const int W = tile + 2 * halo;
+    const int npg = smg.nstokes < g->npol ? smg.nstokes : g->npol;
+    const size_t n = size_t(ni) * size_t(npg) * 2 * size_t(W) * size_t(W);
+    if (ni) {
+      P->slots_g = P->alloc<float>(n);
+      if (!P->slots_g) (void)hipGetLastError();
+    }
+  }
   rc = build_bins(P, P->pd, P->vd, P->place_d, chunk, &P->recs_d, &P->items_d, &ni, &nr, st);
   if (rc) return bail(rc);
   P->pd.nitems_b = ni;
@@ -1302,6 +1363,11 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
     return bail(fail(DDF_EHIP, "device query failed"));
   P->nwg = unsigned(o.workgroups > 0 ? o.workgroups : prop.multiProcessorCount * 4);
+  if (o.workgroups > 0) {
+    for (unsigned& x : P->nwg_tiles) x = unsigned(o.workgroups);
+  } else if (batched_tile_wgs(P->pg, P->pd, P->nwg_tiles)) {
+    return bail(fail(DDF_EHIP, "occupancy query failed"));
+  }
   if (hipStreamSynchronize(st) != hipSuccess) return bail(fail(DDF_EHIP, "plan build failed"));
   *out = P;
   return DDF_OK;
@@ -1314,10 +1380,16 @@ extern "C" int ddf_plan_grid(ddf_plan* P, float* vis, int32_t dopsf, void* strea
   if (vis) v.vis = reinterpret_cast<float2*>(vis);
   BatchParams p = P->pg;
   p.dopsf = dopsf;
+  if (prepare_jones(P, st)) return fail(DDF_EHIP, "Jones preparation failed");
   if (batched_reset(P->queues, 2, st)) return fail(DDF_EHIP, "launch failed");
+  P->mark(DDF_PASS_GRID_A, 0, st);
   if (p.nitems_a && batched_grid_a(p, v, P->place_g, P->V, P->queues, P->nwg, st)) return fail(DDF_EHIP, "grid pass A launch failed");
-  if (p.nitems_b && batched_grid_b(p, P->bplace_g, P->V, P->items_g, P->recs_g, P->queues + 1, P->nwg, st))
+  P->mark(DDF_PASS_GRID_A, 1, st);
+  P->mark(DDF_PASS_GRID_B, 0, st);
+  if (p.nitems_b && batched_grid_b(p, P->bplace_g, P->V, P->items_g, P->recs_g, P->queues + 1, P->nwg_tiles[0], P->slots_g,
+                                     P->tile_first_g, P->tile_count_g, st))
     return fail(DDF_EHIP, "grid pass B launch failed");
+  P->mark(DDF_PASS_GRID_B, 1, st);
   return DDF_OK;
 }
 
@@ -1326,29 +1398,60 @@ extern "C" int ddf_plan_degrid(ddf_plan* P, float* vis, void* stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
   VisP v = P->vd;
   if (vis) v.vis = reinterpret_cast<float2*>(vis);
+  if (prepare_jones(P, st)) return fail(DDF_EHIP, "Jones preparation failed");
   if (batched_reset(P->queues + 2, 1, st)) return fail(DDF_EHIP, "launch failed");
-  if (P->pd.nitems_b && batched_degrid_a(P->pd, P->bplace_d, P->M, P->items_d, P->recs_d, P->queues + 2, P->nwg, st))
+  P->mark(DDF_PASS_DEGRID_A, 0, st);
+  if (P->pd.nitems_b && batched_degrid_a(P->pd, P->bplace_d, P->M, P->items_d, P->recs_d, P->queues + 2, P->nwg_tiles[1], st))
     return fail(DDF_EHIP, "degrid pass A launch failed");
+  P->mark(DDF_PASS_DEGRID_A, 1, st);
+  P->mark(DDF_PASS_DEGRID_B, 0, st);
   if (P->nvl && batched_degrid_b(P->pd, v, P->M, P->vrc, P->vblk, P->nvl, st))
     return fail(DDF_EHIP, "degrid pass B launch failed");
+  P->mark(DDF_PASS_DEGRID_B, 1, st);
   return DDF_OK;
 }
 
 extern "C" int ddf_plan_degrid_adjoint(ddf_plan* P, const float* vis_bar, void* stream) {
   if (!P || !vis_bar) return fail(DDF_EINVAL, "null argument");
   hipStream_t st = static_cast<hipStream_t>(stream);
+  if (prepare_jones(P, st)) return fail(DDF_EHIP, "Jones preparation failed");
   if (batched_reset(P->queues + 2, 2, st)) return fail(DDF_EHIP, "launch failed");
   const float2* vb = reinterpret_cast<const float2*>(vis_bar);
+  P->mark(DDF_PASS_ADJ_B, 0, st);
   if (P->pd.nitems_a && batched_degrid_b_adjoint(P->pd, P->vd, P->place_d, vb, P->M, P->queues + 2, P->nwg, st))
     return fail(DDF_EHIP, "degrid adjoint pass B launch failed");
-  if (P->pd.nitems_b && batched_degrid_a_adjoint(P->pd, P->bplace_d, P->M, P->items_d, P->recs_d, P->queues + 3, P->nwg, st))
+  P->mark(DDF_PASS_ADJ_B, 1, st);
+  P->mark(DDF_PASS_ADJ_A, 0, st);
+  if (P->pd.nitems_b && batched_degrid_a_adjoint(P->pd, P->bplace_d, P->M, P->items_d, P->recs_d, P->queues + 3, P->nwg_tiles[2], st))
     return fail(DDF_EHIP, "degrid adjoint pass A launch failed");
+  P->mark(DDF_PASS_ADJ_A, 1, st);
   return DDF_OK;
 }
 
 extern "C" int ddf_plan_info(const ddf_plan* P, int64_t* out, int32_t n) {
   if (!P || !out) return fail(DDF_EINVAL, "null argument");
   for (int i = 0; i < n && i < 8; ++i) out[i] = P->stats[i];
+  return DDF_OK;
+}
+
+extern "C" int ddf_plan_timing(ddf_plan* P, int32_t enable) {
+  if (!P) return fail(DDF_EINVAL, "null plan");
+  if (enable && !P->ev[0][0]) {
+    for (auto& e : P->ev)
+      for (hipEvent_t& x : e) HIP_TRY(hipEventCreate(&x));
+  }
+  P->timing = enable ? 1 : 0;
+  return DDF_OK;
+}
+
+extern "C" int ddf_plan_pass_ms(ddf_plan* P, float* ms, int32_t n) {
+  if (!P || !ms) return fail(DDF_EINVAL, "null argument");
+  for (int i = 0; i < n; ++i) {
+    ms[i] = -1.f;
+    if (i >= DDF_PLAN_NPASS || !P->ev_set[i]) continue;
+    HIP_TRY(hipEventSynchronize(P->ev[i][1]));
+    HIP_TRY(hipEventElapsedTime(&ms[i], P->ev[i][0], P->ev[i][1]));
+  }
   return DDF_OK;
 }
 

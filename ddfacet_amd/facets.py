@@ -217,6 +217,41 @@ class DeviceFacetSet:
                 "tile", "facet_group"]
         return dict(zip(keys, list(a)))
 
+    def plan_timing(self, enable=True):
+        """Bracket every batched pass with HIP events on its launch stream (ddf_plan_timing)."""
+        check(lib().ddf_plan_timing(self._plan, int(bool(enable))))
+        self._timing = bool(enable)
+
+    def capture_cycle(self, stream=None):
+        """One major cycle of the batched path (zero the grids, grid, degrid) captured into a HIP
+        graph on a side stream; ``graph.replay()`` runs it on the current stream's device.  The
+        passes contain no memset nodes (work queues are reset by a kernel, grids by a fill
+        kernel) and make no device queries, so the capture is complete.  Pass timing is off
+        while capturing."""
+        if getattr(self, "_plan", None) is None:
+            self.plan_batched(stream=stream)
+        timing = getattr(self, "_timing", False)
+        if timing:
+            self.plan_timing(False)
+        cur = torch.cuda.current_stream() if stream is None else stream
+        side = torch.cuda.Stream(device=self.dev)
+        side.wait_stream(cur)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=side):
+            self.zero_grids()
+            self.grid_batched(stream=side)
+            self.degrid_batched(stream=side)
+        cur.wait_stream(side)
+        if timing:
+            self.plan_timing(True)
+        return g
+
+    def pass_ms(self):
+        """Duration (ms) of the latest run of each batched pass, None for passes not run."""
+        a = (C.c_float * len(abi.PASS_NAMES))()
+        check(lib().ddf_plan_pass_ms(self._plan, a, len(abi.PASS_NAMES)))
+        return {k: (float(x) if x >= 0 else None) for k, x in zip(abi.PASS_NAMES, a)}
+
     def destroy_plan(self):
         if getattr(self, "_plan", None) is not None:
             lib().ddf_plan_destroy(self._plan)

@@ -179,7 +179,8 @@ typedef struct ddf_plan_opts {
   int32_t blocks_per_item; /* BDA blocks per gridding work item (0: 128)                        */
   int32_t group;           /* facets per lane group in gridding (0: auto, <= 64)                */
   int32_t precision;       /* 0 FP64 phase base + FP32 sincos + FP64 block sums; 1 FP64; 2 FP32 */
-  int32_t workgroups;      /* persistent workgroups per launch (0: 4 per compute unit)          */
+  int32_t workgroups;      /* persistent workgroups per launch (0: 4 per compute unit for the
+                              per-block passes, as many as fit on the device for the tile passes) */
 } ddf_plan_opts;
 
 int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, const ddf_vis* vdegrid,
@@ -200,6 +201,19 @@ int ddf_plan_degrid_adjoint(ddf_plan* plan, const float* vis_bar, void* stream);
 /* Counters: [0] grid records, [1] degrid records, [2] grid work items, [3] degrid work items,
  * [4] degrid visibilities, [5] device bytes held, [6] tile, [7] facet group.                    */
 int ddf_plan_info(const ddf_plan* plan, int64_t* out, int32_t n);
+/* Per-pass timing, a measurement hook with no reference counterpart: with timing on, every pass
+ * of ddf_plan_grid / ddf_plan_degrid / ddf_plan_degrid_adjoint is bracketed by HIP events on the
+ * launch stream; ddf_plan_pass_ms waits for the latest recording of each pass and writes its
+ * duration in ms (-1 for a pass not yet run) for passes 0..n-1 in the order below.             */
+#define DDF_PASS_GRID_A 0   /* grid pass A: per-block phased Jones-corrected block sums        */
+#define DDF_PASS_GRID_B 1   /* grid pass B: LDS-tiled CF stamps into the facet grids           */
+#define DDF_PASS_DEGRID_A 2 /* degrid pass A: LDS-staged CF gathers of the model grids          */
+#define DDF_PASS_DEGRID_B 3 /* degrid pass B: per-visibility sum over facets, subtract        */
+#define DDF_PASS_ADJ_B 4    /* adjoint of degrid pass B                                        */
+#define DDF_PASS_ADJ_A 5    /* adjoint of degrid pass A                                        */
+#define DDF_PLAN_NPASS 6
+int ddf_plan_timing(ddf_plan* plan, int32_t enable);
+int ddf_plan_pass_ms(ddf_plan* plan, float* ms, int32_t n);
 int ddf_plan_destroy(ddf_plan* plan);
 
 /* ---- facet image side of the major cycle ---------------------------------------------------

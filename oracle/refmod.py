@@ -18,7 +18,9 @@ def available():
     return bool(glob.glob(os.path.join(_HERE, "_ref", "_pyGridderSmearPols3x*.so")))
 
 
-def module():
+def module(nsem=7):
+    """The reference module with `nsem` row semaphores registered (the reference deployment
+    uses NSemaphores = 3373, ClassFacetMachine.py:156-176); the first call fixes the set."""
     global _MOD
     if _MOD is None:
         paths = glob.glob(os.path.join(_HERE, "_ref", "_pyGridderSmearPols3x*.so"))
@@ -27,7 +29,7 @@ def module():
         spec = importlib.util.spec_from_file_location("_pyGridderSmearPols3x", paths[0])
         mod = importlib.util.module_from_spec(spec)
         spec.loader.exec_module(mod)
-        names = ["/ddfgpu_ref_%d_%d" % (os.getpid(), i) for i in range(7)]
+        names = ["/ddfgpu_ref_%d_%d" % (os.getpid(), i) for i in range(int(nsem))]
         mod.pySetSemaphores(names)
         atexit.register(mod.pyDeleteSemaphore)
         _MOD = mod
