@@ -272,7 +272,7 @@ def run_autograd(a, gd_steps=20):
     # adjoint check on random vectors
     xr = torch.randn(shape, generator=gen, device=dev)
     yr = torch.randn(data.shape, generator=gen, device=dev, dtype=torch.complex64)
-    lhs = torch.sum(op.forward(xr).double().conj() * yr.to(torch.complex128)).real
+    lhs = torch.sum(op.forward(xr).to(torch.complex128).conj() * yr.to(torch.complex128)).real
     rhs = torch.sum(xr.double() * op.adjoint(yr).double())
     dot_rel = float(abs(lhs - rhs) / max(abs(float(lhs)), 1e-30))
     # step 1 / L, L = 2 lambda_max(A^H W A) by power iteration
@@ -289,18 +289,18 @@ def run_autograd(a, gd_steps=20):
         loss = chi2(x, op, data, w)
         loss.backward()
         with torch.no_grad():
-            x -= alpha * x.grad
+            x.sub_(alpha * x.grad)
         x.grad = None
         return loss
 
-    first = float(gd_step())
+    first = float(gd_step().detach())
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(gd_steps):
         last = gd_step()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / gd_steps
-    last = float(last)
+    last = float(last.detach())
     R, C = P.uvw.shape[0], P.freqs.shape[0]
     out = {
         "metric": "chi^2 gradient-descent steps through the differentiable degridder, 25 facets x 4 bands",
@@ -329,11 +329,12 @@ def main():
                          "launches against the autotuned batched cycle (eager and graph-replayed); "
                          "autograd25: 20 chi^2 gradient-descent steps through the differentiable "
                          "degridder on 25 facets x 4 bands")
+    ap.add_argument("--tile", type=int, default=0, help="uv tile edge of the batched plan (0: the plan default)")
     ap.add_argument("--graph", action="store_true",
                     help="also time the batched cycle replayed from a HIP graph")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="run only a device copy of 1 GiB (reads 1 GiB, writes 1 GiB), to scale the "
-                         "FETCH_SIZE / WRITE_SIZE counters of a profiler run")
+                    help="run only an elementwise copy of 1 GiB (reads 1 GiB, writes 1 GiB), to scale "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a profiler run")
     a = ap.parse_args()
     if a.copy_calibration:
         import torch
@@ -341,9 +342,10 @@ def main():
         y = torch.empty_like(x)
         torch.cuda.synchronize()
         for _ in range(3):
-            y.copy_(x)
+            torch.mul(x, 1.0, out=y)  # an elementwise kernel: 1 GiB read, 1 GiB written per launch
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4, "launches": 3}))
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4, "launches": 3,
+                          "kernel": "elementwise mul by 1 (3 launches after the fill of x)"}))
         return
     if a.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md for how "
@@ -381,7 +383,7 @@ def main():
 
     stream = torch.cuda.current_stream()
     if a.mode == "batched":
-        fs.plan_batched(stream=stream)
+        fs.plan_batched(tile=a.tile, stream=stream)
         fs.plan_timing(True)
         info = fs.plan_info()
         log("plan: %s" % info)

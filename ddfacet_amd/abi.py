@@ -154,6 +154,21 @@ class ddf_image_facet(C.Structure):
     ]
 
 
+class ddf_cf_spec(C.Structure):
+    _fields_ = [
+        ("nw", C.c_int32),
+        ("overs", C.c_int32),
+        ("nfacets", C.c_int32),
+        ("pad_", C.c_int32),
+        ("sups", P),
+        ("wlist", P),
+        ("coefs", P),
+        ("tapers", P),
+        ("ref_wave", C.c_double),
+        ("lrad", C.c_double),
+    ]
+
+
 PRECISION = {"mixed": 0, "fp64": 1, "fp32": 2}
 
 EXTRA_SIGNATURES = {
@@ -163,6 +178,7 @@ EXTRA_SIGNATURES = {
     "ddf_plan_grid": [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p],
     "ddf_plan_degrid": [C.c_void_p, C.c_void_p, C.c_void_p],
     "ddf_plan_info": [C.c_void_p, C.POINTER(C.c_int64), C.c_int32],
+    "ddf_cf_build": [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p],
     "ddf_plan_timing": [C.c_void_p, C.c_int32],
     "ddf_plan_pass_ms": [C.c_void_p, C.POINTER(C.c_float), C.c_int32],
     "ddf_plan_destroy": [C.c_void_p],

@@ -3,9 +3,10 @@
 cfg1  plain single facet: 27 antennas in a 20 km disk, 351 baselines, 100 x 10 s, 16 channels
       1.2-1.6 GHz, 512 x 512 grid at 2", Nw = 8, S = 7, OverS = 11, identity Jones, one BDA
       block per (row, channel), 10 point sources + noise, 1% flags.
-cfg3  precision study: the cfg1 array with 256 channels over 0.9-1.67 GHz, one facet of 301 px at
-      2" padded to 512 (Nw = 8, S = 7, OverS = 11), 20 point sources at pixel centres 4-12 px
-      inside the facet edges, visibilities from the FP64 DFT, no noise, no flags.
+cfg3  precision study: the cfg1 array with 256 channels over 0.9-1.67 GHz, 3 x 3 facets of 297 px
+      at 2" padded to 525 (EstimateNpix of a 301 px facet with Padding 1.7, the cfg1 grid size;
+      Nw = 8, S = 7, OverS = 11), 20 point sources at pixel centres 4-12 px from the internal
+      facet boundaries, visibilities from the FP64 DFT, no noise, no flags.
 cfg4  uneven batched workload: the cfg1 array (27 antennas, 20 km disk, 100 x 10 s, 16 channels
       1.2-1.6 GHz) with baseline-dependent averaging k_t = k_f in {1, 2, 4, 8, 16} by baseline-length
       quintile (longest -> 1, shortest -> 16; blocks cut at the 4-channel band edges, so the
@@ -162,7 +163,7 @@ def cfg2_visibilities(uvw, freqs, sky, device="cpu"):
 
 
 def small_cycle(seed=5, nfx=3, m=243, n=147, cell_arcsec=20.0, nant=7, ntime=10, nchan=6, nsrc=6,
-                jones=False, nw=4, sups=(7, 7, 9, 9), overs=11, flag_frac=0.0, bda="pervis"):
+                jones=False, nw=4, sups=(7, 7, 9, 9), overs=11, flag_frac=0.0, bda="pervis", nband=1):
     """A small faceted major cycle: nfx x nfx facets on an m x m image (padded facets n px),
     point sources at pixel centres, visibilities from the FP64 DFT.  Returns (Problem, geometry)."""
     from .imaging import FacetGeometry
@@ -190,39 +191,46 @@ def small_cycle(seed=5, nfx=3, m=243, n=147, cell_arcsec=20.0, nant=7, ntime=10,
         f = make_cf_facet(l0, m0, n, cell_arcsec, freqs, nw, list(sups), overs, wmax)
         f.kms_dir = i
         facets.append(f)
+    chan_map = (np.arange(nchan) * nband // nchan).astype(np.int32)
     if bda == "pervis":
         bmap = bdamod.bda_map_per_visibility(obs.a0, obs.a1, nchan)
     else:
-        bmap = bdamod.bda_map_factors(obs.a0, obs.a1, nchan, lambda key: (2, 2))
+        bmap = bdamod.bda_map_factors(obs.a0, obs.a1, nchan, lambda key: (2, 2), chan_map=chan_map)
     jd = None
     if jones:
         jd = dict(mode=1, interp=0, apply_amp=1, apply_phase=1, scale=0, calib_error=0.0,
                   kms=synth.random_jones(2, geom.nf, nant, 2, seed + 4, 0.05, 0.2),
                   kms_time=np.minimum((obs.tidx * 2) // ntime, 1).astype(np.int32),
                   kms_chan=np.minimum((np.arange(nchan) * 2) // nchan, 1).astype(np.int32))
-    chan_map = np.zeros(nchan, np.int32)
     P = Problem(uvw=obs.uvw, vis=vis, flags=flags, weights=weights, freqs=freqs, a0=obs.a0, a1=obs.a1,
                 chan_map=chan_map, chan_map_degrid=chan_map.copy(), bda_grid=bmap, bda_degrid=bmap,
-                facets=facets, npix=n, nband=1, npol=1, incr=np.array([-cell, cell]),
+                facets=facets, npix=n, nband=nband, npol=1, incr=np.array([-cell, cell]),
                 corr_codes=[9, 10, 11, 12], stokes_codes=[1], jones=jd)
     P.extra.update(sky=sky, src_pix=(X, Y), cell=cell, image_npix=m, name="small_cycle")
     return P, geom
 
 
-def cfg3(nsrc=20, m=301, n=512, nchan=256, cell_arcsec=2.0, log=print):
-    """The precision-study workload (see the module docstring).  Returns (Problem, geometry)."""
+def cfg3(nsrc=20, nfx=3, facet=297, n=525, nchan=256, cell_arcsec=2.0, chan_block=1, log=print):
+    """The precision-study workload (see the module docstring).  Returns (Problem, geometry).
+    chan_block > 1 replaces the per-visibility blocks by blocks of one row and chan_block
+    channels (the gridders then walk chan_block channels per block)."""
     from .imaging import FacetGeometry
     t0 = time.time()
     freqs = np.linspace(0.9e9, 1.67e9, nchan)
     obs = synth.make_observation("disk", 27, 100, 10.0, freqs, 45.0, -30.7, 3000, diameter_m=20000.0)
     cell = cell_arcsec * synth.ARCSEC
-    geom = FacetGeometry(m, 1, m, n, cell, mixing_width=4.0)
+    m = nfx * facet
+    geom = FacetGeometry(m, nfx, facet, n, cell, mixing_width=4.0)
     rng = np.random.default_rng(3001)
-    side = rng.integers(0, 4, nsrc)
+    # sources 4-12 px from an internal facet boundary (or from the image edge with one facet)
+    bounds = [facet * k for k in range(1, nfx)] or [0, m]
+    b = np.array(bounds)[rng.integers(0, len(bounds), nsrc)]
+    side = np.where(b == 0, 1, np.where(b == m, -1, rng.choice([-1, 1], nsrc)))
+    across = b + side * rng.integers(4, 13, nsrc) - (side < 0)
     along = rng.integers(12, m - 12, nsrc)
-    inset = rng.integers(4, 13, nsrc)
-    X = np.where(side == 0, inset, np.where(side == 1, m - 1 - inset, along))
-    Y = np.where(side == 2, inset, np.where(side == 3, m - 1 - inset, along))
+    vertical = rng.integers(0, 2, nsrc).astype(bool)
+    X = np.where(vertical, across, along)
+    Y = np.where(vertical, along, across)
     flux = rng.uniform(0.5, 2.0, nsrc)
     sky = synth.Sky(flux, (X - m // 2) * cell, (Y - m // 2) * cell)
     vis = synth.predict_point_sources(obs.uvw, freqs, sky).numpy()
@@ -230,17 +238,24 @@ def cfg3(nsrc=20, m=301, n=512, nchan=256, cell_arcsec=2.0, log=print):
     flags = np.zeros((R, nchan, 4), np.uint8)
     weights = np.ones((R, nchan), np.float32)
     wmax = float(np.abs(obs.uvw[:, 2]).max()) + 1e-3
-    f = make_cf_facet(0.0, 0.0, n, cell_arcsec, freqs, 8, [7] * 8, 11, wmax)
-    f.kms_dir = 0
-    bmap = bdamod.bda_map_per_visibility(obs.a0, obs.a1, nchan)
+    facets = []
+    for i in range(geom.nf):
+        l0, m0 = geom.lm[i]
+        f = make_cf_facet(l0, m0, n, cell_arcsec, freqs, 8, [7] * 8, 11, wmax)
+        f.kms_dir = i
+        facets.append(f)
+    if chan_block > 1:
+        bmap = bdamod.bda_map_factors(obs.a0, obs.a1, nchan, lambda key: (1, chan_block))
+    else:
+        bmap = bdamod.bda_map_per_visibility(obs.a0, obs.a1, nchan)
     chan_map = np.zeros(nchan, np.int32)
     P = Problem(uvw=obs.uvw, vis=vis, flags=flags, weights=weights, freqs=freqs, a0=obs.a0, a1=obs.a1,
                 chan_map=chan_map, chan_map_degrid=chan_map.copy(), bda_grid=bmap, bda_degrid=bmap,
-                facets=[f], npix=n, nband=1, npol=1, incr=np.array([-cell, cell]),
+                facets=facets, npix=n, nband=1, npol=1, incr=np.array([-cell, cell]),
                 corr_codes=[9, 10, 11, 12], stokes_codes=[1])
     P.extra.update(sky=sky, src_pix=(X, Y), cell=cell, image_npix=m, name="cfg3",
                    equidistant=chan_equidistant(freqs))
-    log("cfg3 built in %.1fs: %d rows x %d chan" % (time.time() - t0, R, nchan))
+    log("cfg3 built in %.1fs: %d rows x %d chan, %d facets" % (time.time() - t0, R, nchan, geom.nf))
     return P, geom
 
 

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void plan_place(BatchParams p, const VisP v, u
   for (int64_t t = tid; t < total; t += int64_t(gridDim.x) * blockDim.x) {
     const int64_t b = t / p.nf;
     const int f = int(t - b * p.nf);
-    const FacetDev& fd = p.facets[f];
+    const FacetDev fd = gload(p.facets + f);
     uint64_t rec = 0;
     BlockRec br = read_block(v, b);
     const int nch = br.ch1 - br.ch0;
@@ -110,12 +110,13 @@ __global__ __launch_bounds__(256) void plan_place(BatchParams p, const VisP v, u
           const int64_t row = br.rows[r];
           if (row < 0 || row >= v.nrow) continue;
           const double* uvw = v.uvw + 3 * row;
+        const DDF_GLOBAL double* guvw = gptr(v.uvw) + 3 * row;
           for (int c = br.ch0; c < br.ch1; ++c) {
-            if (v.flags[(size_t(row) * v.nchan + c) * v.ncorr]) continue;
-            su += shifted(uvw[0], uvw[2], fd.cu);
-            sv += shifted(uvw[1], uvw[2], fd.cv);
-            sw += uvw[2];
-            fm += v.freqs[c];
+            if (gptr(v.flags)[(size_t(row) * v.nchan + c) * v.ncorr]) continue;
+            su += shifted(guvw[0], guvw[2], fd.cu);
+            sv += shifted(guvw[1], guvw[2], fd.cv);
+            sw += guvw[2];
+            fm += gptr(v.freqs)[c];
             ++cnt;
           }
         }
@@ -123,15 +124,16 @@ __global__ __launch_bounds__(256) void plan_place(BatchParams p, const VisP v, u
           su /= cnt; sv /= cnt; sw /= cnt; fm /= cnt;
         }
       } else {
-        for (int c = br.ch0; c < br.ch1; ++c) fm += v.freqs[c];
+        for (int c = br.ch0; c < br.ch1; ++c) fm += gptr(v.freqs)[c];
         fm /= double(nch);
         for (int r = 0; r < br.nr; ++r) {
           const int64_t row = br.rows[r];
           if (row < 0 || row >= v.nrow) continue;
           const double* uvw = v.uvw + 3 * row;
-          su += shifted(uvw[0], uvw[2], fd.cu);
-          sv += shifted(uvw[1], uvw[2], fd.cv);
-          sw += uvw[2];
+        const DDF_GLOBAL double* guvw = gptr(v.uvw) + 3 * row;
+          su += shifted(guvw[0], guvw[2], fd.cu);
+          sv += shifted(guvw[1], guvw[2], fd.cv);
+          sw += guvw[2];
           ++cnt;
         }
         if (cnt > 0) {
@@ -178,10 +180,21 @@ __global__ __launch_bounds__(256) void plan_fill(BatchParams p, int64_t nblocks,
   }
 }
 
-__global__ __launch_bounds__(256) void plan_gather_place(const uint64_t* place, const uint32_t* recs, int64_t n,
-                                                         uint64_t* bplace) {
-  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x)
-    bplace[k] = place[recs[k]];
+// Stamp records of the binned records (record k of the bin list): see Stamp.
+__global__ __launch_bounds__(256) void plan_stamp_records(BatchParams p, const uint64_t* place, const uint32_t* recs,
+                                                          int64_t n, uint64_t* bstamp) {
+  const int W = p.tile + 2 * p.halo;
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t t = recs[k];
+    const Unpacked u = unpack_place(place[t]);
+    const int f = int(t % uint32_t(p.nf));
+    const int S = plane_support(p, u.plane);
+    const int sup = (S - 1) / 2;
+    const int x0 = (u.locx / p.tile) * p.tile - p.halo, y0 = (u.locy / p.tile) * p.tile - p.halo;
+    const int base = (u.locy - sup - y0) * W + (u.locx - sup - x0);
+    const int64_t cfoff = gptr(gload(p.facets + f).cf_offset)[u.plane] + int64_t(u.io * p.overs + u.jo) * S * S;
+    bstamp[k] = uint64_t(uint32_t(base) & 0xFFFF) | (uint64_t(S & 0x1F) << 16) | (uint64_t(uint32_t(cfoff)) << 32);
+  }
 }
 
 // Prepared Jones table of the batched plan: entry (t, dir, ant, chan) of a raw table
@@ -259,7 +272,10 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
   const int per_f = nbp + 2 * p.nband;
   double* s_acc = lds;
   double* s_chan = lds + p.nf * per_f;
-  const int nchan_acc = p.chan_lds ? G * 2 * v.nchan : 0;
+  // one [2][nchan] row per facet of the group, padded to an odd number of doubles so that the
+  // lanes (facets) of a wavefront fall on different LDS banks
+  const int cs = 2 * v.nchan + 1;
+  const int nchan_acc = p.chan_lds ? G * cs : 0;
   for (int i = threadIdx.x; i < p.nf * per_f + nchan_acc; i += blockDim.x) lds[i] = 0.;
   __syncthreads();
   const int slot = threadIdx.x / G;
@@ -274,9 +290,10 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
     for (int i = threadIdx.x; i < nchan_acc; i += blockDim.x) {
       const double x = s_chan[i];
       if (x == 0.) continue;
-      const int fg = i / (2 * v.nchan);
+      const int fg = i / cs;
+      const int k = i - fg * cs;
       const int f = group * G + fg;
-      if (f < p.nf) atomicAdd(&p.facets[f].sum_jones_chan[i - fg * 2 * v.nchan], x);
+      if (f < p.nf && k < 2 * v.nchan) atomic_add_g(gptr_w(gload(p.facets + f).sum_jones_chan) + k, x);
       s_chan[i] = 0.;
     }
   };
@@ -300,7 +317,7 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
     for (int64_t b = b0 + slot; fok && b < b1; b += slots) {
       const uint64_t rec = place[b * p.nf + f];
       if (!place_valid(rec)) continue;
-      const FacetDev& fd = p.facets[f];
+      const FacetDev fd = gload(p.facets + f);
       BlockRec br = read_block(v, b);
       const int64_t mid = br.rows[br.nr / 2];
       const bool mid_ok = mid >= 0 && mid < v.nrow;
@@ -323,9 +340,9 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
         for (int r = 0; r < br.nr; ++r) {
           const int64_t row = br.rows[r];
           if (row < 0 || row >= v.nrow) continue;
-          const double* q = v.uvw + 3 * row;
+          const DDF_GLOBAL double* q = gptr(v.uvw) + 3 * row;
           for (int c = br.ch0; c < br.ch1; ++c) {
-            if (v.flags[(size_t(row) * v.nchan + c) * v.ncorr]) continue;
+            if (gptr(v.flags)[(size_t(row) * v.nchan + c) * v.ncorr]) continue;
             su += shifted(q[0], q[2], fd.cu);
             sv += shifted(q[1], q[2], fd.cv);
             sw += q[2];
@@ -336,7 +353,7 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
         jblk = jones_at<true>(jp, mid_ok ? mid : 0, (br.ch0 + br.ch1) / 2, uvwm, p.wl_mean);
         BBblk = jblk.BB;
       }
-      double* sch = s_chan + lf * 2 * v.nchan;
+      double* sch = s_chan + lf * cs;
       if (JMODE == 1) {
         // Per-visibility Jones without ScaleJones depends only on the key (baseline, Jones time,
         // Jones channel), so J0^H (sum_v meas_v w_v) J1 over a run of one key equals the sum of
@@ -374,13 +391,14 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
           const int64_t row = br.rows[r];
           if (row < 0 || row >= v.nrow) continue;
           const double* uvw = v.uvw + 3 * row;
+        const DDF_GLOBAL double* guvw = gptr(v.uvw) + 3 * row;
           ChanPhasor<PREC> ph;
-          ph.row(-2. * kPi * (uvw[0] * fd.l0 + uvw[1] * fd.m0 + uvw[2] * fd.n0) / kC);
+          ph.row(-2. * kPi * (guvw[0] * fd.l0 + guvw[1] * fd.m0 + guvw[2] * fd.n0) / kC);
           const int k_t = jp.kms ? jp.ktime[row] : -1;
           const int b_t = jp.beam ? jp.btime[row] : -1;
           for (int c = br.ch0; c < br.ch1; ++c) {
             const size_t vc = size_t(row) * v.nchan + c;
-            if (v.flags[vc * v.ncorr]) continue;
+            if (gptr(v.flags)[vc * v.ncorr]) continue;
             const int k_c = jp.kms ? jp.kchan[c] : -1, b_c = jp.beam ? jp.bchan[c] : -1;
             if (!any || k_t != kt || k_c != kc || b_t != bt || b_c != bc) {
               if (any) flush();
@@ -388,13 +406,13 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
               kt = k_t; kc = k_c; bt = b_t; bc = b_c;
               any = true;
             }
-            const double fw = double(v.weights[vc]);
+            const double fw = double(gptr(v.weights)[vc]);
             const double fwd = fw * dec * dec;
             if (p.dopsf) {
               rfw += fw;
             } else {
               const cd wt = ph.at(v, c) * fw;
-              const float2* vp = v.vis + vc * v.ncorr;
+              const DDF_GLOBAL float2* vp = gptr(v.vis) + vc * v.ncorr;
               if (READ == 4) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) racc[q] = racc[q] + cd{vp[q].x, vp[q].y} * wt;
@@ -414,8 +432,8 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
               atomicAdd(&sch[c], x0);
               atomicAdd(&sch[v.nchan + c], x1);
             } else {
-              atomicAdd(&fd.sum_jones_chan[c], x0);
-              atomicAdd(&fd.sum_jones_chan[v.nchan + c], x1);
+              atomic_add_g(gptr_w(fd.sum_jones_chan) + c, x0);
+              atomic_add_g(gptr_w(fd.sum_jones_chan) + (v.nchan + c), x1);
             }
             swt += fw;
             ssw += fwd;
@@ -429,18 +447,19 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
         const int64_t row = br.rows[r];
         if (row < 0 || row >= v.nrow) continue;
         const double* uvw = v.uvw + 3 * row;
+        const DDF_GLOBAL double* guvw = gptr(v.uvw) + 3 * row;
         ChanPhasor<PREC> ph;
-        ph.row(-2. * kPi * (uvw[0] * fd.l0 + uvw[1] * fd.m0 + uvw[2] * fd.n0) / kC);
+        ph.row(-2. * kPi * (guvw[0] * fd.l0 + guvw[1] * fd.m0 + guvw[2] * fd.n0) / kC);
         for (int c = br.ch0; c < br.ch1; ++c) {
           const size_t vc = size_t(row) * v.nchan + c;
-          if (v.flags[vc * v.ncorr]) continue;
-          const double fw = double(v.weights[vc]);
+          if (gptr(v.flags)[vc * v.ncorr]) continue;
+          const double fw = double(gptr(v.weights)[vc]);
           const double fwd = fw * dec * dec;
           cd meas[4];
           if (p.dopsf) {
             meas[0] = cd{dec, 0}; meas[1] = cd{0, 0}; meas[2] = cd{0, 0}; meas[3] = cd{dec, 0};
           } else {
-            const float2* vp = v.vis + vc * v.ncorr;
+            const DDF_GLOBAL float2* vp = gptr(v.vis) + vc * v.ncorr;
             if (READ == 4) {
 #pragma unroll
               for (int q = 0; q < 4; ++q) meas[q] = cd{vp[q].x, vp[q].y};
@@ -467,17 +486,17 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
             sj += jj.BB * fwd;
             const double x = double(npg) * (jj.BB * fwd);
             if (p.chan_lds) atomicAdd(&sch[c], x);
-            else atomicAdd(&fd.sum_jones_chan[c], x);
+            else atomic_add_g(gptr_w(fd.sum_jones_chan) + c, x);
           }
           if (JMODE == 2) {
             const double x = double(npg) * (fwd * BBblk);
             if (p.chan_lds) atomicAdd(&sch[c], x);
-            else atomicAdd(&fd.sum_jones_chan[c], x);
+            else atomic_add_g(gptr_w(fd.sum_jones_chan) + c, x);
           }
           if (JMODE != 0) {
             const double x = double(npg) * fwd;
             if (p.chan_lds) atomicAdd(&sch[v.nchan + c], x);
-            else atomicAdd(&fd.sum_jones_chan[v.nchan + c], x);
+            else atomic_add_g(gptr_w(fd.sum_jones_chan) + (v.nchan + c), x);
           }
           if (READ == 4) {
 #pragma unroll
@@ -526,8 +545,8 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
     if (x == 0.) continue;
     const int f = i / per_f;
     const int k = i - f * per_f;
-    if (k < nbp) atomicAdd(&p.facets[f].sumwt[k], x);
-    else atomicAdd(&p.facets[f].sum_jones[k - nbp], x);
+    if (k < nbp) atomic_add_g(gptr_w(gload(p.facets + f).sumwt) + k, x);
+    else atomic_add_g(gptr_w(gload(p.facets + f).sum_jones) + (k - nbp), x);
   }
 }
 
@@ -566,25 +585,22 @@ __device__ __forceinline__ double wave_total(double x) {
   return __longlong_as_double((long long)(uint64_t(lane_u32(uint32_t(b), 63)) | (uint64_t(lane_u32(uint32_t(b >> 32), 63)) << 32)));
 }
 
-// A record's stamp in tile coordinates: cell c (row c / S, column c % S of the S x S stamp) sits at
-// tile offset base + (c / S) * W + c % S; its CF value is cfp[c].
+// Stamp record of a binned (block, facet) record (plan_stamp_records): base | S << 16 | cfoff << 32,
+// with base the tile offset of the stamp's first cell (row (locy - sup) - y0, column
+// (locx - sup) - x0 of the (T + 2 h)^2 tile image) and cfoff the element offset of the record's
+// oversampled S x S sub-kernel in its facet's CF buffer.  Cell c of the stamp (row c / S, column
+// c % S) sits at tile offset base + (c / S) W + c % S and takes CF value cf[cfoff + c].
 struct Stamp {
-  const float2* cfp;
+  const DDF_GLOBAL float2* cfp;
   int S, S2, base;
   float invS;
 };
-__device__ __forceinline__ Stamp stamp_of(uint64_t rec, const int* s_sup, const int64_t* s_off, const float2* cf,
-                                          int overs, int W, int x0, int y0) {
+__device__ __forceinline__ Stamp stamp_of(uint64_t srec, const DDF_GLOBAL float2* cf) {
   Stamp s;
-  const int plane = int((rec >> 32) & 0xFF);
-  s.S = uniform_i32(s_sup[plane]);
-  const int64_t coff = uniform_i64(s_off[plane]);
-  const int io = int((rec >> 40) & 0xFF), jo = int((rec >> 48) & 0xFF);
-  const int locx = int(rec & 0xFFFF), locy = int((rec >> 16) & 0xFFFF);
-  const int sup = (s.S - 1) / 2;
+  s.base = int(srec & 0xFFFF);
+  s.S = int((srec >> 16) & 0x1F);
   s.S2 = s.S * s.S;
-  s.cfp = cf + coff + int64_t(io * overs + jo) * s.S2;
-  s.base = (locy - sup - y0) * W + (locx - sup - x0);
+  s.cfp = cf + uint32_t(srec >> 32);
   s.invS = 1.f / float(s.S);
   return s;
 }
@@ -594,104 +610,132 @@ __device__ __forceinline__ int stamp_cell(const Stamp& s, int c, int W) {
   const int sy = int((float(c) + 0.5f) * s.invS);
   return c < s.S2 ? sy * W + (c - sy * s.S) : -1;
 }
+// The first 256 cells of a stamp (the whole stamp for S <= 15): per lane 4 cells, CF values
+// loaded unconditionally (lanes past the stamp read cell 0) so that the loads of the next record
+// can be issued before the LDS work of the current one.
+struct StampCells {
+  float2 w[4];
+  int off[4];
+};
+__device__ __forceinline__ void fetch_cells(const Stamp& s, int lane, int W, StampCells& sc) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int c = 64 * r + lane;
+    sc.off[r] = stamp_cell(s, c, W);
+    sc.w[r] = ld2(s.cfp + (c < s.S2 ? c : 0));
+  }
+}
 
 // ---- cycle: grid pass B (LDS-tiled scatter) ------------------------------------------------------
-// One workgroup per bin item.  Each wavefront loads 64 records (placement, value) at once, then
-// stamps them one after the other with the record in scalar registers; lanes are stamp cells
-// (4 at a time per lane, CF loads issued together), and every polarisation of the record goes
-// into LDS (separate real / imaginary planes) with LDS float atomics.  The tile is then added
-// into the facet grid with global float atomics (non-zero cells only).
+// One single-wavefront workgroup per bin item, with a private LDS tile image (T + 2 h)^2 per
+// polarisation, (re, im) interleaved.  The wavefront loads 64 records (stamp record, value) at a
+// time, then adds their stamps one after the other with the record in scalar registers; lanes are
+// stamp cells (4 per lane) and the CF values of the next record are loaded while the current one
+// is added.  The adds are plain LDS read-modify-writes: the cells of one stamp are distinct lanes'
+// and a wavefront's LDS operations complete in issue order, so no atomics are needed (LDS float
+// atomics serialise over the lanes).  The tile then goes to its slot (or, without slots, into the
+// facet grid with global float atomics on the non-zero cells).
 template <bool CONJ>
-__global__ __launch_bounds__(256) void grid_pass_b(BatchParams p, const uint64_t* bplace, const float2* V,
-                                                   const BinItem* items, const uint32_t* recs, uint32_t* queue,
-                                                   float* slots) {
-  extern __shared__ float tile[];  // [npg][2][W * W]
-  __shared__ int s_sup[256];
-  __shared__ int64_t s_off[256];
+__device__ __forceinline__ void stamp_add(float2* tile, int WW, int npg, const Stamp& s, const StampCells& sc,
+                                          const float* vr, const float* vi) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if (sc.off[r] < 0) continue;
+    const float wx = sc.w[r].x, wy = CONJ ? -sc.w[r].y : sc.w[r].y;
+    float2* t = tile + s.base + sc.off[r];
+#pragma unroll
+    for (int ip = 0; ip < 4; ++ip) {
+      if (ip < npg) {
+        float2 x = t[ip * WW];
+        x.x += vr[ip] * wx - vi[ip] * wy;
+        x.y += vr[ip] * wy + vi[ip] * wx;
+        t[ip * WW] = x;
+      }
+    }
+  }
+}
+
+template <bool CONJ>
+__global__ __launch_bounds__(64) void grid_pass_b(BatchParams p, const uint64_t* bstamp, const float2* V,
+                                                  const BinItem* items, const uint32_t* recs, uint32_t* queue,
+                                                  float2* slots) {
+  extern __shared__ float lds_f[];
+  float2* tile = reinterpret_cast<float2*>(lds_f);  // [npg][W * W]
   __shared__ uint32_t s_item;
   const int W = p.tile + 2 * p.halo;
   const int WW = W * W;
   const int npg = p.nstokes < p.npol ? p.nstokes : p.npol;
-  const int nplanes = 2 * p.nw;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int nwave = blockDim.x >> 6;
-  for (int i = threadIdx.x; i < nplanes; i += blockDim.x) s_sup[i] = plane_support(p, i);
+  const int lane = threadIdx.x;
   while (true) {
-    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
+    if (lane == 0) s_item = atomicAdd(queue, 1u);
     __syncthreads();
     const uint32_t it = s_item;
     if (it >= p.nitems_b) break;
     const BinItem bi = items[it];
-    const FacetDev& fd = p.facets[bi.facet];
-    const float2* cf = fd.cf;
-    for (int i = threadIdx.x; i < npg * 2 * WW; i += blockDim.x) tile[i] = 0.f;
-    for (int i = threadIdx.x; i < nplanes; i += blockDim.x) s_off[i] = fd.cf_offset[i];
-    __syncthreads();
+    const FacetDev fd = gload(p.facets + bi.facet);
+    const DDF_GLOBAL float2* cf = gptr(fd.cf);
+    for (int i = lane; i < npg * WW; i += 64) tile[i] = make_float2(0.f, 0.f);
     const int x0 = bi.tx * p.tile - p.halo, y0 = bi.ty * p.tile - p.halo;
-    for (uint32_t k0 = bi.begin + 64u * wave; k0 < bi.end; k0 += 64u * nwave) {
+    for (uint32_t k0 = bi.begin; k0 < bi.end; k0 += 64u) {
       const uint32_t k = k0 + lane;
       const bool ok = k < bi.end;
-      const uint64_t my_rec = ok ? bplace[k] : 0;
+      const uint64_t my_rec = ok ? bstamp[k] : 0;
       const uint32_t my_t = ok ? recs[k] : 0;
       float2 my_v[4];
 #pragma unroll
       for (int ip = 0; ip < 4; ++ip) my_v[ip] = (ok && ip < npg) ? V[size_t(my_t) * p.nstokes + ip] : make_float2(0.f, 0.f);
       const int cnt = int(min(64u, bi.end - k0));
+      Stamp cur = stamp_of(lane_u64(my_rec, 0), cf);
+      StampCells cc;
+      fetch_cells(cur, lane, W, cc);
       for (int j = 0; j < cnt; ++j) {
-        const Stamp s = stamp_of(lane_u64(my_rec, j), s_sup, s_off, cf, p.overs, W, x0, y0);
+        Stamp nxt = cur;
+        StampCells nc = cc;
+        if (j + 1 < cnt) {
+          nxt = stamp_of(lane_u64(my_rec, j + 1), cf);
+          fetch_cells(nxt, lane, W, nc);
+        }
         float vr[4], vi[4];
 #pragma unroll
         for (int ip = 0; ip < 4; ++ip) {
           vr[ip] = lane_f32(my_v[ip].x, j);
           vi[ip] = lane_f32(my_v[ip].y, j);
         }
-        for (int c0 = 0; c0 < s.S2; c0 += 256) {
-          float2 w[4];
-          int off[4];
+        stamp_add<CONJ>(tile, WW, npg, cur, cc, vr, vi);
+        // stamps wider than 16 x 16: the cells past the first 256, without prefetch
+        for (int c0 = 256; c0 < cur.S2; c0 += 256) {
+          StampCells xc;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int c = c0 + 64 * r + lane;
-            off[r] = stamp_cell(s, c, W);
-            w[r] = off[r] >= 0 ? s.cfp[c] : make_float2(0.f, 0.f);
+            xc.off[r] = stamp_cell(cur, c, W);
+            xc.w[r] = ld2(cur.cfp + (c < cur.S2 ? c : 0));
           }
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            if (off[r] < 0) continue;
-            const float wx = w[r].x, wy = CONJ ? -w[r].y : w[r].y;
-            float* t = tile + s.base + off[r];
-#pragma unroll
-            for (int ip = 0; ip < 4; ++ip) {
-              if (ip < npg) {
-                atomicAdd(t + (2 * ip) * WW, vr[ip] * wx - vi[ip] * wy);
-                atomicAdd(t + (2 * ip + 1) * WW, vr[ip] * wy + vi[ip] * wx);
-              }
-            }
-          }
+          stamp_add<CONJ>(tile, WW, npg, cur, xc, vr, vi);
         }
+        cur = nxt;
+        cc = nc;
       }
     }
-    __syncthreads();
     if (slots) {
       // the whole tile into this item's slot with plain stores (grid_slot_reduce adds the slots)
-      float4* dst = reinterpret_cast<float4*>(slots + size_t(it) * npg * 2 * WW);
+      float4* dst = reinterpret_cast<float4*>(slots + size_t(it) * npg * WW);
       const float4* src = reinterpret_cast<const float4*>(tile);
-      const int n4 = (npg * 2 * WW) / 4;
-      for (int i = threadIdx.x; i < n4; i += blockDim.x) dst[i] = src[i];
-      for (int i = 4 * n4 + threadIdx.x; i < npg * 2 * WW; i += blockDim.x)
-        slots[size_t(it) * npg * 2 * WW + i] = tile[i];
+      const int n4 = (npg * WW) / 2;
+      for (int i = lane; i < n4; i += 64) dst[i] = src[i];
+      if ((npg * WW) & 1 && lane == 0) slots[size_t(it) * npg * WW + npg * WW - 1] = tile[npg * WW - 1];
     } else {
-      float* grid = reinterpret_cast<float*>(fd.grid);
-      for (int i = threadIdx.x; i < npg * WW; i += blockDim.x) {
+      DDF_GLOBAL float* grid = gptr_w(reinterpret_cast<float*>(fd.grid));
+      for (int i = lane; i < npg * WW; i += 64) {
         const int ip = i / WW;
         const int rem = i - ip * WW;
-        const float re = tile[2 * ip * WW + rem], im = tile[(2 * ip + 1) * WW + rem];
-        if (re == 0.f && im == 0.f) continue;
+        const float2 x = tile[i];
+        if (x.x == 0.f && x.y == 0.f) continue;
         const int gy = y0 + rem / W, gx = x0 + rem % W;
         if (gx < 0 || gy < 0 || gx >= p.nx || gy >= p.ny) continue;
         const size_t gi = ((size_t(bi.band) * p.npol + ip) * p.ny + gy) * p.nx + gx;
-        atomicAdd(&grid[2 * gi], re);
-        atomicAdd(&grid[2 * gi + 1], im);
+        atomic_add_g(grid + 2 * gi, x.x);
+        atomic_add_g(grid + 2 * gi + 1, x.y);
       }
     }
     __syncthreads();
@@ -699,9 +743,9 @@ __global__ __launch_bounds__(256) void grid_pass_b(BatchParams p, const uint64_t
 }
 
 // grid[f][band][pol][y][x] += the tile slots covering (y, x), one thread per grid cell: the
-// tiles ty with ty * T - h <= y < ty * T + T + h (at most two per axis, since h < T), each with
+// tiles ty with ty * T - h <= y < ty * T + T + h (at most two per axis when h < T), each with
 // its consecutive work items, in item order.  No atomics; the same sum order on every run.
-__global__ __launch_bounds__(256) void grid_slot_reduce(BatchParams p, const float* slots, const uint32_t* tile_first,
+__global__ __launch_bounds__(256) void grid_slot_reduce(BatchParams p, const float2* slots, const uint32_t* tile_first,
                                                         const uint32_t* tile_count) {
   const int T = p.tile, h = p.halo;
   const int W = T + 2 * h;
@@ -724,93 +768,104 @@ __global__ __launch_bounds__(256) void grid_slot_reduce(BatchParams p, const flo
         const int lx = x - (tx * T - h);
         const int64_t tid = ((int64_t(f) * p.nband + band) * p.ntiles_y + ty) * p.ntiles_x + tx;
         const uint32_t first = tile_first[tid], n = tile_count[tid];
-        const float* t = slots + size_t(first) * npg * 2 * WW + size_t(2 * ip) * WW + size_t(ly) * W + lx;
-        for (uint32_t k = 0; k < n; ++k, t += size_t(npg) * 2 * WW) {
-          re += t[0];
-          im += t[WW];
+        const float2* t = slots + size_t(first) * npg * WW + size_t(ip) * WW + size_t(ly) * W + lx;
+        for (uint32_t k = 0; k < n; ++k, t += size_t(npg) * WW) {
+          const float2 v = *t;
+          re += v.x;
+          im += v.y;
         }
       }
     }
     if (re != 0.f || im != 0.f) {
-      float2* g = reinterpret_cast<float2*>(p.facets[f].grid) + ((size_t(band) * p.npol + ip) * p.ny + y) * p.nx + x;
-      const float2 o = *g;
-      *g = make_float2(o.x + re, o.y + im);
+      DDF_GLOBAL float2* g = gptr_w(reinterpret_cast<float2*>(gload(p.facets + f).grid)) +
+                             ((size_t(band) * p.npol + ip) * p.ny + y) * p.nx + x;
+      const float2 o = ld2(g);
+      st2(g, make_float2(o.x + re, o.y + im));
     }
   }
 }
 
 // ---- cycle: degrid pass A (LDS-staged gather) --------------------------------------------------
 // The model tile of a bin item is staged in LDS (separate real / imaginary planes); each record's
-// stamp is gathered by one wavefront: FP32 products (grid x CF) summed in FP64, per lane and then
-// across the wavefront (degridder.h:202-212 sums the FP32 products in double).
-__global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64_t* bplace, float2* M,
+// stamp is gathered by one wavefront (next record's CF values loaded ahead): FP32 products
+// (grid x CF) summed in FP64, per lane and then across the wavefront (degridder.h:202-212 sums
+// the FP32 products in double).
+__device__ __forceinline__ void stamp_gather(const float* tile, int WW, int ns, const Stamp& s, const StampCells& sc,
+                                             double* sr, double* si) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if (sc.off[r] < 0) continue;
+    const float* t = tile + s.base + sc.off[r];
+#pragma unroll
+    for (int ip = 0; ip < 4; ++ip) {
+      if (ip < ns) {
+        const float mr = t[(2 * ip) * WW], mi = t[(2 * ip + 1) * WW];
+        sr[ip] += double(mr * sc.w[r].x - mi * sc.w[r].y);
+        si[ip] += double(mr * sc.w[r].y + mi * sc.w[r].x);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64_t* bstamp, float2* M,
                                                      const BinItem* items, const uint32_t* recs, uint32_t* queue) {
   extern __shared__ float tile[];  // [nstokes][2][W * W]
-  __shared__ int s_sup[256];
-  __shared__ int64_t s_off[256];
   __shared__ uint32_t s_item;
   const int W = p.tile + 2 * p.halo;
   const int WW = W * W;
   const int ns = p.nstokes;
-  const int nplanes = 2 * p.nw;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nwave = blockDim.x >> 6;
-  for (int i = threadIdx.x; i < nplanes; i += blockDim.x) s_sup[i] = plane_support(p, i);
   while (true) {
     if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
     __syncthreads();
     const uint32_t it = s_item;
     if (it >= p.nitems_b) break;
     const BinItem bi = items[it];
-    const FacetDev& fd = p.facets[bi.facet];
-    const float2* cf = fd.cf;
+    const FacetDev fd = gload(p.facets + bi.facet);
+    const DDF_GLOBAL float2* cf = gptr(fd.cf);
     const int x0 = bi.tx * p.tile - p.halo, y0 = bi.ty * p.tile - p.halo;
-    const float2* model = reinterpret_cast<const float2*>(fd.model);
+    const DDF_GLOBAL float2* model = gptr(reinterpret_cast<const float2*>(fd.model));
     for (int i = threadIdx.x; i < ns * WW; i += blockDim.x) {
       const int ip = i / WW;
       const int rem = i - ip * WW;
       const int gy = y0 + rem / W, gx = x0 + rem % W;
       float2 m = make_float2(0.f, 0.f);
       if (gx >= 0 && gy >= 0 && gx < p.nx && gy < p.ny)
-        m = model[((size_t(bi.band) * p.npol + ip) * p.ny + gy) * p.nx + gx];
+        m = ld2(model + ((size_t(bi.band) * p.npol + ip) * p.ny + gy) * p.nx + gx);
       tile[2 * ip * WW + rem] = m.x;
       tile[(2 * ip + 1) * WW + rem] = m.y;
     }
-    for (int i = threadIdx.x; i < nplanes; i += blockDim.x) s_off[i] = fd.cf_offset[i];
     __syncthreads();
     for (uint32_t k0 = bi.begin + 64u * wave; k0 < bi.end; k0 += 64u * nwave) {
       const uint32_t k = k0 + lane;
       const bool ok = k < bi.end;
-      const uint64_t my_rec = ok ? bplace[k] : 0;
+      const uint64_t my_rec = ok ? bstamp[k] : 0;
       const uint32_t my_t = ok ? recs[k] : 0;
       float2 res[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
       const int cnt = int(min(64u, bi.end - k0));
+      Stamp cur = stamp_of(lane_u64(my_rec, 0), cf);
+      StampCells cc;
+      fetch_cells(cur, lane, W, cc);
       for (int j = 0; j < cnt; ++j) {
-        const Stamp s = stamp_of(lane_u64(my_rec, j), s_sup, s_off, cf, p.overs, W, x0, y0);
+        Stamp nxt = cur;
+        StampCells nc = cc;
+        if (j + 1 < cnt) {
+          nxt = stamp_of(lane_u64(my_rec, j + 1), cf);
+          fetch_cells(nxt, lane, W, nc);
+        }
         double sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
-        for (int c0 = 0; c0 < s.S2; c0 += 256) {
-          float2 w[4];
-          int off[4];
+        stamp_gather(tile, WW, ns, cur, cc, sr, si);
+        for (int c0 = 256; c0 < cur.S2; c0 += 256) {
+          StampCells xc;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int c = c0 + 64 * r + lane;
-            off[r] = stamp_cell(s, c, W);
-            w[r] = off[r] >= 0 ? s.cfp[c] : make_float2(0.f, 0.f);
+            xc.off[r] = stamp_cell(cur, c, W);
+            xc.w[r] = ld2(cur.cfp + (c < cur.S2 ? c : 0));
           }
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            if (off[r] < 0) continue;
-            const float* t = tile + s.base + off[r];
-#pragma unroll
-            for (int ip = 0; ip < 4; ++ip) {
-              if (ip < ns) {
-                const float mr = t[(2 * ip) * WW], mi = t[(2 * ip + 1) * WW];
-                sr[ip] += double(mr * w[r].x - mi * w[r].y);
-                si[ip] += double(mr * w[r].y + mi * w[r].x);
-              }
-            }
-          }
+          stamp_gather(tile, WW, ns, cur, xc, sr, si);
         }
 #pragma unroll
         for (int ip = 0; ip < 4; ++ip) {
@@ -819,6 +874,8 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
             if (lane == j) res[ip] = make_float2(float(a), float(b));
           }
         }
+        cur = nxt;
+        cc = nc;
       }
       if (ok) {
 #pragma unroll
@@ -841,7 +898,8 @@ __global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v
     const int64_t b = vblk[k];
     if (row < 0 || row >= v.nrow || c < 0 || c >= v.nchan) continue;
     const double* uvw = v.uvw + 3 * row;
-    const double fr = v.freqs[c];
+        const DDF_GLOBAL double* guvw = gptr(v.uvw) + 3 * row;
+    const double fr = gptr(v.freqs)[c];
     cd acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     BlockRec br;
     bool have_br = false;
@@ -861,7 +919,7 @@ __global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v
         }
       }
       if (!any) continue;  // dropped record, or a zero model: contributes exactly 0
-      const FacetDev& fd = p.facets[f];
+      const FacetDev fd = gload(p.facets + f);
       M2 cv;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -874,7 +932,7 @@ __global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v
       jp.bdir = fd.beam_dir;
       if ((JMODE == 2 || p.dp.enabled) && !have_br) {
         br = read_block(v, b);
-        for (int cc = br.ch0; cc < br.ch1; ++cc) fm += v.freqs[cc];
+        for (int cc = br.ch0; cc < br.ch1; ++cc) fm += gptr(v.freqs)[cc];
         fm /= double(br.ch1 - br.ch0);
         have_br = true;
       }
@@ -893,7 +951,7 @@ __global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v
         dp.m0 = fd.dec_m0;
         if (mid >= 0 && mid < v.nrow) dec = decorr_factor(dp, v.uvw, fm, mid);
       }
-      const double angle = 2. * kPi * (uvw[0] * fd.l0 + uvw[1] * fd.m0 + uvw[2] * fd.n0) / kC;
+      const double angle = 2. * kPi * (guvw[0] * fd.l0 + guvw[1] * fd.m0 + guvw[2] * fd.n0) / kC;
       const cd corr = phasor_prec<PREC>(fr, angle) * dec;
       if (JMODE == 1) {
         JonesPair jj = jones_at<true>(jp, row, c, uvw, p.wl_mean);
@@ -902,20 +960,20 @@ __global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[q] = acc[q] + cv.a[q] * corr;
     }
-    float2* vp = v.vis + (size_t(row) * v.nchan + c) * NCORR;
+    DDF_GLOBAL float2* vp = gptr_w(v.vis) + (size_t(row) * v.nchan + c) * NCORR;
     if (NCORR == 4) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const float2 x = vp[q];
-        vp[q] = make_float2(float(double(x.x) - acc[q].re), float(double(x.y) - acc[q].im));
+        const float2 x = ld2(vp + q);
+        st2(vp + q, make_float2(float(double(x.x) - acc[q].re), float(double(x.y) - acc[q].im)));
       }
     } else if (NCORR == 2) {
-      const float2 x0 = vp[0], x1 = vp[1];
-      vp[0] = make_float2(float(double(x0.x) - acc[0].re), float(double(x0.y) - acc[0].im));
-      vp[1] = make_float2(float(double(x1.x) - acc[3].re), float(double(x1.y) - acc[3].im));
+      const float2 x0 = ld2(vp), x1 = ld2(vp + 1);
+      st2(vp, make_float2(float(double(x0.x) - acc[0].re), float(double(x0.y) - acc[0].im)));
+      st2(vp + 1, make_float2(float(double(x1.x) - acc[3].re), float(double(x1.y) - acc[3].im)));
     } else {
-      const float2 x0 = vp[0];
-      vp[0] = make_float2(float(double(x0.x) - acc[0].re), float(double(x0.y) - acc[0].im));
+      const float2 x0 = ld2(vp);
+      st2(vp, make_float2(float(double(x0.x) - acc[0].re), float(double(x0.y) - acc[0].im)));
     }
   }
 }
@@ -946,10 +1004,10 @@ __global__ __launch_bounds__(256) void degrid_adj_b(BatchParams p, const VisP v,
     const int64_t b1 = min(b0 + p.blocks_per_item, v.nblocks);
     for (int64_t b = b0 + slot; active && f < p.nf && b < b1; b += slots) {
       if (!place_valid(place[b * p.nf + f])) continue;
-      const FacetDev& fd = p.facets[f];
+      const FacetDev fd = gload(p.facets + f);
       BlockRec br = read_block(v, b);
       double fm = 0;
-      for (int c = br.ch0; c < br.ch1; ++c) fm += v.freqs[c];
+      for (int c = br.ch0; c < br.ch1; ++c) fm += gptr(v.freqs)[c];
       fm /= double(br.ch1 - br.ch0);
       const int64_t mid = br.rows[br.nr / 2];
       const bool mid_ok = mid >= 0 && mid < v.nrow;
@@ -965,7 +1023,8 @@ __global__ __launch_bounds__(256) void degrid_adj_b(BatchParams p, const VisP v,
         const int64_t row = br.rows[r];
         if (row < 0 || row >= v.nrow) continue;
         const double* uvw = v.uvw + 3 * row;
-        const double angle = 2. * kPi * (uvw[0] * fd.l0 + uvw[1] * fd.m0 + uvw[2] * fd.n0) / kC;
+        const DDF_GLOBAL double* guvw = gptr(v.uvw) + 3 * row;
+        const double angle = 2. * kPi * (guvw[0] * fd.l0 + guvw[1] * fd.m0 + guvw[2] * fd.n0) / kC;
         for (int c = br.ch0; c < br.ch1; ++c) {
           const float2* vp = vbar + (size_t(row) * v.nchan + c) * NCORR;
           M2 m;
@@ -977,7 +1036,7 @@ __global__ __launch_bounds__(256) void degrid_adj_b(BatchParams p, const VisP v,
           } else {
             m.a[0] = cd{vp[0].x, vp[0].y}; m.a[1] = cd{0, 0}; m.a[2] = cd{0, 0}; m.a[3] = cd{0, 0};
           }
-          const cd cc = conj(phasor_prec<PREC>(v.freqs[c], angle)) * dec;
+          const cd cc = conj(phasor_prec<PREC>(gptr(v.freqs)[c], angle)) * dec;
           if (JMODE == 1) {
             JonesPair jj = jones_at<true>(jp, row, c, uvw, p.wl_mean);
             m = mul(mul(herm(jj.J0), m), jj.J1);
@@ -1087,7 +1146,7 @@ static void launch_a_j(const BatchParams& p, const VisP& v, const uint64_t* plac
 
 int batched_grid_a(const BatchParams& p, const VisP& v, const uint64_t* place, float2* V, uint32_t* queue,
                    unsigned nwg, hipStream_t st) {
-  size_t lds = (size_t(p.nf) * (p.nband * p.npol + 2 * p.nband) + (p.chan_lds ? size_t(p.group) * 2 * v.nchan : 0)) * 8;
+  size_t lds = (size_t(p.nf) * (p.nband * p.npol + 2 * p.nband) + (p.chan_lds ? size_t(p.group) * (2 * v.nchan + 1) : 0)) * 8;
   if (p.read == 4) launch_a_j<4>(p, v, place, V, queue, lds, nwg, st);
   else if (p.read == 2) launch_a_j<2>(p, v, place, V, queue, lds, nwg, st);
   else launch_a_j<1>(p, v, place, V, queue, lds, nwg, st);
@@ -1096,12 +1155,12 @@ int batched_grid_a(const BatchParams& p, const VisP& v, const uint64_t* place, f
 
 // Persistent grid size: `nwg` when given, else as many workgroups as fit on the device at once.
 template <typename K>
-static unsigned persistent_wgs(K kernel, size_t lds, unsigned nwg) {
+static unsigned persistent_wgs(K kernel, size_t lds, unsigned nwg, int block = 256) {
   if (nwg) return nwg;
   int dev = 0, ncu = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 1024;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, lds) != hipSuccess || per < 1) per = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, lds) != hipSuccess || per < 1) per = 1;
   return unsigned(ncu * per);
 }
 
@@ -1110,9 +1169,9 @@ static unsigned persistent_wgs(K kernel, size_t lds, unsigned nwg) {
 int batched_tile_wgs(const BatchParams& pg, const BatchParams& pd, unsigned out[3]) {
   const int Wg = pg.tile + 2 * pg.halo, Wd = pd.tile + 2 * pd.halo;
   const int npg = pg.nstokes < pg.npol ? pg.nstokes : pg.npol;
-  out[0] = persistent_wgs(grid_pass_b<false>, size_t(npg) * Wg * Wg * 2 * sizeof(float), 0);
+  out[0] = persistent_wgs(grid_pass_b<false>, size_t(npg) * Wg * Wg * sizeof(float2), 0, 64);
   out[1] = persistent_wgs(degrid_pass_a, size_t(pd.nstokes) * Wd * Wd * 2 * sizeof(float), 0);
-  out[2] = persistent_wgs(grid_pass_b<true>, size_t(pd.nstokes) * Wd * Wd * 2 * sizeof(float), 0);
+  out[2] = persistent_wgs(grid_pass_b<true>, size_t(pd.nstokes) * Wd * Wd * sizeof(float2), 0, 64);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
@@ -1121,16 +1180,17 @@ int batched_grid_b(const BatchParams& p, const uint64_t* place, const float2* V,
                    const uint32_t* tile_count, hipStream_t st) {
   const int W = p.tile + 2 * p.halo;
   const int npg = p.nstokes < p.npol ? p.nstokes : p.npol;
-  size_t lds = size_t(npg) * W * W * 2 * sizeof(float);
-  nwg = persistent_wgs(grid_pass_b<false>, lds, nwg);
-  hipLaunchKernelGGL(grid_pass_b<false>, dim3(nwg), dim3(256), lds, st, p, place, V, items, recs, queue, slots);
+  size_t lds = size_t(npg) * W * W * sizeof(float2);
+  hipLaunchKernelGGL(grid_pass_b<false>, dim3(nwg), dim3(64), lds, st, p, place, V, items, recs, queue,
+                     reinterpret_cast<float2*>(slots));
   if (hipGetLastError() != hipSuccess) return DDF_EHIP;
   if (slots) {
     const int64_t total = int64_t(p.nf) * p.nband * npg * p.ny * p.nx;
     int64_t nb = (total + 255) / 256;
     if (nb > 8192) nb = 8192;
     if (nb < 1) nb = 1;
-    hipLaunchKernelGGL(grid_slot_reduce, dim3(unsigned(nb)), dim3(256), 0, st, p, slots, tile_first, tile_count);
+    hipLaunchKernelGGL(grid_slot_reduce, dim3(unsigned(nb)), dim3(256), 0, st, p, reinterpret_cast<const float2*>(slots),
+                       tile_first, tile_count);
   }
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
@@ -1139,12 +1199,11 @@ int batched_grid_b(const BatchParams& p, const uint64_t* place, const float2* V,
 int batched_degrid_a_adjoint(const BatchParams& p, const uint64_t* place, const float2* Mbar, const BinItem* items,
                              const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st) {
   const int W = p.tile + 2 * p.halo;
-  size_t lds = size_t(p.nstokes) * W * W * 2 * sizeof(float);
+  size_t lds = size_t(p.nstokes) * W * W * sizeof(float2);
   BatchParams q = p;
   q.npol = p.npol;
-  nwg = persistent_wgs(grid_pass_b<true>, lds, nwg);
-  hipLaunchKernelGGL(grid_pass_b<true>, dim3(nwg), dim3(256), lds, st, q, place, Mbar, items, recs, queue,
-                     static_cast<float*>(nullptr));
+  hipLaunchKernelGGL(grid_pass_b<true>, dim3(nwg), dim3(64), lds, st, q, place, Mbar, items, recs, queue,
+                     static_cast<float2*>(nullptr));
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
@@ -1219,11 +1278,12 @@ int batched_degrid_b_adjoint(const BatchParams& p, const VisP& v, const uint64_t
 
 namespace ddf {
 
-int batched_gather_place(const uint64_t* place, const uint32_t* recs, int64_t n, uint64_t* bplace, hipStream_t st) {
-  if (n <= 0) return DDF_OK;
+int batched_stamp_records(const BatchParams& p, const uint64_t* place, const uint32_t* recs, int64_t n,
+                          uint64_t* bstamp, hipStream_t st) {
   int64_t nb = (n + 255) / 256;
   if (nb > 65536) nb = 65536;
-  hipLaunchKernelGGL(plan_gather_place, dim3(unsigned(nb)), dim3(256), 0, st, place, recs, n, bplace);
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(plan_stamp_records, dim3(unsigned(nb)), dim3(256), 0, st, p, place, recs, n, bstamp);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 

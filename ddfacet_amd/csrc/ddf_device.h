@@ -11,6 +11,48 @@
 
 namespace ddf {
 
+// Global-memory views of pointers.  Pointers reached through a struct (kernel-argument structs,
+// facet descriptors) are generic to the compiler, and generic loads are flat loads, which count
+// on both the vector-memory and the LDS counters, so every LDS wait also waits for them (and the
+// reverse).  Every buffer these kernels touch is device global memory; these views give the loads,
+// stores and atomics their global forms (gptr for reads, gptr_w for writes and atomics).
+#define DDF_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const DDF_GLOBAL T* gptr(const T* p) {
+  return (const DDF_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ DDF_GLOBAL T* gptr_w(T* p) {
+  return (DDF_GLOBAL T*)p;
+}
+// A struct read from global memory (word by word: structs do not copy across address spaces).
+template <typename T>
+__device__ __forceinline__ T gload(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "gload reads whole 32-bit words");
+  T out;
+  const DDF_GLOBAL uint32_t* src = gptr(reinterpret_cast<const uint32_t*>(p));
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+  for (int i = 0; i < int(sizeof(T) / 4); ++i) dst[i] = src[i];
+  return out;
+}
+// Complex values in global memory (the vector classes do not copy across address spaces).
+__device__ __forceinline__ float2 ld2(const DDF_GLOBAL float2* p) {
+  const DDF_GLOBAL float* q = reinterpret_cast<const DDF_GLOBAL float*>(p);
+  return make_float2(q[0], q[1]);
+}
+__device__ __forceinline__ void st2(DDF_GLOBAL float2* p, float2 v) {
+  DDF_GLOBAL float* q = reinterpret_cast<DDF_GLOBAL float*>(p);
+  q[0] = v.x;
+  q[1] = v.y;
+}
+// Relaxed device-scope atomic add on global memory (returns nothing).
+template <typename T>
+__device__ __forceinline__ void atomic_add_g(DDF_GLOBAL T* p, T v) {
+  (void)__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+
 constexpr double kC = 299792458.0;
 constexpr double kPi = 3.141592653589793238462643383279502884197;
 constexpr int kWave = 64;
@@ -104,7 +146,7 @@ __device__ inline M2 give_jones(const float2* tab, const int* dims, int it, int 
   ant = clampi(ant, 0, dims[2] - 1);
   ich = clampi(ich, 0, dims[3] - 1);
   size_t off = ((size_t(it) * dims[1] + dir) * size_t(dims[2]) + ant) * size_t(dims[3]) + ich;
-  const float2* p = tab + off * 4;
+  const DDF_GLOBAL float2* p = gptr(tab) + off * 4;
   M2 out;
   if (interp == 0) {
 #pragma unroll
@@ -118,7 +160,7 @@ __device__ inline M2 give_jones(const float2* tab, const int* dims, int it, int 
 #pragma unroll
   for (int q = 0; q < 4; ++q) out.a[q] = cd{0, 0};
   for (int d = 0; d < ncoefs; ++d) {
-    double c = coefs[d];
+    double c = gptr(coefs)[d];
     if (c == 0) continue;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -174,7 +216,7 @@ __device__ __forceinline__ M2 prepared_jones(const double2* tab, const int* dims
   dir = clampi(dir, 0, dims[1] - 1);
   ant = clampi(ant, 0, dims[2] - 1);
   ich = clampi(ich, 0, dims[3] - 1);
-  const double2* p = tab + (((size_t(it) * dims[2] + ant) * size_t(dims[3]) + ich) * size_t(dims[1]) + dir) * 4;
+  const DDF_GLOBAL double2* p = gptr(tab) + (((size_t(it) * dims[2] + ant) * size_t(dims[3]) + ich) * size_t(dims[1]) + dir) * 4;
   M2 out;
 #pragma unroll
   for (int q = 0; q < 4; ++q) out.a[q] = cd{p[q].x, p[q].y};
@@ -186,11 +228,11 @@ __device__ __forceinline__ M2 prepared_jones(const double2* tab, const int* dims
 template <bool PREP = false>
 __device__ inline JonesPair jones_at(const JonesP& jp, int64_t row, int chan, const double* uvw,
                                      double wl_mean) {
-  int a0 = jp.a0[row], a1 = jp.a1[row];
+  int a0 = gptr(jp.a0)[row], a1 = gptr(jp.a1)[row];
   if (PREP) {
     JonesPair r;
     if (jp.pk) {
-      const int t = jp.ktime[row], c = jp.kchan[chan];
+      const int t = gptr(jp.ktime)[row], c = gptr(jp.kchan)[chan];
       r.J0 = prepared_jones(jp.pk, jp.kdims, t, a0, jp.kdir, c);
       r.J1 = prepared_jones(jp.pk, jp.kdims, t, a1, jp.kdir, c);
       if (jp.scale) {
@@ -198,12 +240,12 @@ __device__ inline JonesPair jones_at(const JonesP& jp, int64_t row, int chan, co
         scale_jones(jp, r.J1, uvw, wl_mean);
       }
       if (jp.pb) {
-        const int tb = jp.btime[row], cb = jp.bchan[chan];
+        const int tb = gptr(jp.btime)[row], cb = gptr(jp.bchan)[chan];
         r.J0 = mul(r.J0, prepared_jones(jp.pb, jp.bdims, tb, a0, jp.bdir, cb));
         r.J1 = mul(r.J1, prepared_jones(jp.pb, jp.bdims, tb, a1, jp.bdir, cb));
       }
     } else {
-      const int tb = jp.btime[row], cb = jp.bchan[chan];
+      const int tb = gptr(jp.btime)[row], cb = gptr(jp.bchan)[chan];
       r.J0 = prepared_jones(jp.pb, jp.bdims, tb, a0, jp.bdir, cb);
       r.J1 = prepared_jones(jp.pb, jp.bdims, tb, a1, jp.bdir, cb);
     }
@@ -213,12 +255,12 @@ __device__ inline JonesPair jones_at(const JonesP& jp, int64_t row, int chan, co
   }
   M2 J0 = unity(), J1 = unity();
   if (jp.beam) {
-    int t = jp.btime[row], c = jp.bchan[chan];
+    int t = gptr(jp.btime)[row], c = gptr(jp.bchan)[chan];
     J0 = give_jones(jp.beam, jp.bdims, t, a0, jp.bdir, c, jp.interp, jp.coefs, jp.ncoefs);
     J1 = give_jones(jp.beam, jp.bdims, t, a1, jp.bdir, c, jp.interp, jp.coefs, jp.ncoefs);
   }
   if (jp.kms) {
-    int t = jp.ktime[row], c = jp.kchan[chan];
+    int t = gptr(jp.ktime)[row], c = gptr(jp.kchan)[chan];
     M2 K0 = give_jones(jp.kms, jp.kdims, t, a0, jp.kdir, c, jp.interp, jp.coefs, jp.ncoefs);
     M2 K1 = give_jones(jp.kms, jp.kdims, t, a1, jp.kdir, c, jp.interp, jp.coefs, jp.ncoefs);
     norm_jones(jp, K0, uvw, wl_mean);
@@ -246,9 +288,9 @@ struct JonesCache {
   __device__ __forceinline__ void reset() { a0 = a1 = kt = kc = bt = bc = -1; }
   __device__ __forceinline__ const JonesPair& get(const JonesP& jp, int64_t row, int chan, const double* uvw,
                                                    double wl_mean) {
-    const int r0 = jp.a0[row], r1 = jp.a1[row];
-    const int k_t = jp.kms ? jp.ktime[row] : -1, k_c = jp.kms ? jp.kchan[chan] : -1;
-    const int b_t = jp.beam ? jp.btime[row] : -1, b_c = jp.beam ? jp.bchan[chan] : -1;
+    const int r0 = gptr(jp.a0)[row], r1 = gptr(jp.a1)[row];
+    const int k_t = jp.kms ? gptr(jp.ktime)[row] : -1, k_c = jp.kms ? gptr(jp.kchan)[chan] : -1;
+    const int b_t = jp.beam ? gptr(jp.btime)[row] : -1, b_c = jp.beam ? gptr(jp.bchan)[chan] : -1;
     if (r0 != a0 || r1 != a1 || k_t != kt || k_c != kc || b_t != bt || b_c != bc) {
       cur = jones_at<PREP>(jp, row, chan, uvw, wl_mean);
       a0 = r0; a1 = r1; kt = k_t; kc = k_c; bt = b_t; bc = b_c;
@@ -269,13 +311,13 @@ __device__ inline double decorr_factor(const DecorrP& d, const double* uvw, doub
   double n0 = sqrt(1. - d.l0 * d.l0 - d.m0 * d.m0) - 1.;
   double f = 1.;
   if (d.fsmear) {
-    const double* q = uvw + 3 * idx;
+    const DDF_GLOBAL double* q = gptr(uvw) + 3 * idx;
     double ph = q[0] * d.l0 + q[1] * d.m0 + q[2] * n0;
     double phi = kPi * d.dnu / kC * ph;
     f *= (phi == 0.) ? 1.0 : sin(phi) / phi;
   }
   if (d.tsmear) {
-    const double* q = d.uvw_dt + 3 * idx;
+    const DDF_GLOBAL double* q = gptr(d.uvw_dt) + 3 * idx;
     double dph = (q[0] * d.l0 + q[1] * d.m0 + q[2] * n0) * d.dt;
     double phi = kPi * nu / kC * dph;
     f *= (phi == 0.) ? 1.0 : sin(phi) / phi;

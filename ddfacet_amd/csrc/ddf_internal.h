@@ -26,27 +26,27 @@ struct VisP {
 
 struct BlockRec {
   int ch0, ch1, nr;
-  const int32_t* rows;
+  const DDF_GLOBAL int32_t* rows;
 };
 
 // Block b of the BDA map, with its channel range clamped to [0, nchan] and its row count to the
 // map length, so a malformed map cannot send a kernel outside its buffers.
 __device__ __forceinline__ BlockRec read_block(const VisP& v, int64_t b) {
-  int64_t s = v.bda_start[b];
+  int64_t s = gptr(v.bda_start)[b];
   BlockRec r;
-  int nr = v.bda[2 + b] - 2;
+  int nr = gptr(v.bda)[2 + b] - 2;
   if (s < 0 || s + 2 > v.bda_len) {
     r.ch0 = r.ch1 = 0;
     r.nr = 0;
-    r.rows = v.bda;
+    r.rows = gptr(v.bda);
     return r;
   }
   if (nr < 0) nr = 0;
   if (s + 2 + nr > v.bda_len) nr = int(v.bda_len - s - 2);
-  r.ch0 = clampi(v.bda[s], 0, v.nchan);
-  r.ch1 = clampi(v.bda[s + 1], r.ch0, v.nchan);
+  r.ch0 = clampi(gptr(v.bda)[s], 0, v.nchan);
+  r.ch1 = clampi(gptr(v.bda)[s + 1], r.ch0, v.nchan);
   r.nr = nr;
-  r.rows = v.bda + s + 2;
+  r.rows = gptr(v.bda) + s + 2;
   return r;
 }
 
@@ -84,17 +84,17 @@ struct ChanPhasor {
     last = -2;
   }
   __device__ __forceinline__ cd at(const VisP& v, int c) {
-    if (PREC != 0 || !v.equi) return phasor_prec<PREC>(v.freqs[c], angle);
+    if (PREC != 0 || !v.equi) return phasor_prec<PREC>(gptr(v.freqs)[c], angle);
     if (c == last + 1 && (c & 15) != 0) {
       const float r = cr * sr - ci * si, i = cr * si + ci * sr;
       cr = r;
       ci = i;
     } else {
-      const cd a = phasor(v.freqs[c] * angle);
+      const cd a = phasor(gptr(v.freqs)[c] * angle);
       cr = float(a.re);
       ci = float(a.im);
       if (last == -2) {
-        const cd st = phasor((v.freqs[1] - v.freqs[0]) * angle);
+        const cd st = phasor((gptr(v.freqs)[1] - gptr(v.freqs)[0]) * angle);
         sr = float(st.re);
         si = float(st.im);
       }
@@ -178,7 +178,9 @@ int batched_degrid_a_adjoint(const BatchParams& p, const uint64_t* place, const 
                              const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st);
 
 int batched_tile_wgs(const BatchParams& pg, const BatchParams& pd, unsigned out[3]);
-int batched_gather_place(const uint64_t* place, const uint32_t* recs, int64_t n, uint64_t* bplace, hipStream_t st);
+// stamp records (tile offset | support << 16 | CF offset << 32) of the binned records
+int batched_stamp_records(const BatchParams& p, const uint64_t* place, const uint32_t* recs, int64_t n,
+                          uint64_t* bstamp, hipStream_t st);
 int batched_prepare_jones(const BatchParams& p, const float2* src, double2* dst, int dd, hipStream_t st);
 
 }  // namespace ddf

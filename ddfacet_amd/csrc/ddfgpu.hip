@@ -913,11 +913,11 @@ struct ddf_plan {
   uint64_t* place_d = nullptr;
   uint32_t* recs_g = nullptr;
   uint32_t* recs_d = nullptr;
-  uint64_t* bplace_g = nullptr;  // placement records in bin order
+  uint64_t* bstamp_g = nullptr;  // stamp records in bin order (tile offset, support, CF offset)
   float* slots_g = nullptr;      // one LDS tile image per grid work item (null: atomic flush)
   uint32_t* tile_first_g = nullptr;
   uint32_t* tile_count_g = nullptr;
-  uint64_t* bplace_d = nullptr;
+  uint64_t* bstamp_d = nullptr;
   BinItem* items_g = nullptr;
   BinItem* items_d = nullptr;
   float2* V = nullptr;
@@ -1193,7 +1193,7 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   if (prepare_jones(P, st)) return bail(fail(DDF_EHIP, "Jones preparation failed"));
   ddf_plan_opts o{};
   if (opts) o = *opts;
-  const int tile = o.tile > 0 ? o.tile : 32;
+  const int tile = o.tile > 0 ? o.tile : 16;
   const int chunk = o.chunk > 0 ? o.chunk : 2048;
   const int halo = (maxsup - 1) / 2;
   {
@@ -1206,6 +1206,8 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
     const int np = std::max(std::min(smg.nstokes, g->npol), smd.nstokes);
     if (size_t(np) * W * W * 8 + 4096 > size_t(lds_max))
       return bail(fail(DDF_EINVAL, "uv tile too large for LDS at this support and polarisation count"));
+    // stamp records hold the tile offset in 16 bits and the support in 5
+    if (W > 255 || maxsup > 31) return bail(fail(DDF_EINVAL, "uv tile or CF support too large for the stamp records"));
   }
   int G = o.group;
   if (G <= 0) {
@@ -1287,8 +1289,8 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   P->pg.nitems_b = ni;
   P->stats[0] = nr;
   P->stats[2] = ni;
-  P->bplace_g = P->alloc<uint64_t>(size_t(nr));
-  if (!P->bplace_g || batched_gather_place(P->place_g, P->recs_g, nr, P->bplace_g, st))
+  P->bstamp_g = P->alloc<uint64_t>(size_t(nr));
+  if (!P->bstamp_g || batched_stamp_records(P->pg, P->place_g, P->recs_g, nr, P->bstamp_g, st))
     return bail(fail(DDF_ENOMEM, "plan allocation failed"));
   {
     // tile slots of the gridding items (plain stores + a deterministic reduction instead of
@@ -1306,8 +1308,8 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   P->pd.nitems_b = ni;
   P->stats[1] = nr;
   P->stats[3] = ni;
-  P->bplace_d = P->alloc<uint64_t>(size_t(nr));
-  if (!P->bplace_d || batched_gather_place(P->place_d, P->recs_d, nr, P->bplace_d, st))
+  P->bstamp_d = P->alloc<uint64_t>(size_t(nr));
+  if (!P->bstamp_d || batched_stamp_records(P->pd, P->place_d, P->recs_d, nr, P->bstamp_d, st))
     return bail(fail(DDF_ENOMEM, "plan allocation failed"));
   // pass A layout
   const int64_t bpi = o.blocks_per_item > 0 ? o.blocks_per_item : 128;
@@ -1322,7 +1324,7 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   const size_t per_f = size_t(g->nband) * g->npol + 2 * size_t(g->nband);
   if (size_t(nfacets) * per_f * 8 > 40 * 1024) return bail(fail(DDF_EINVAL, "facets x bands too large for one plan"));
   P->pg.chan_lds = (P->pg.jmode != 0) &&
-                   (size_t(nfacets) * per_f + size_t(G) * 2 * size_t(vgrid->nchan)) * 8 <= 64 * 1024;
+                   (size_t(nfacets) * per_f + size_t(G) * (2 * size_t(vgrid->nchan) + 1)) * 8 <= 64 * 1024;
   // LDS tiles
   const int W = tile + 2 * halo;
   const int npg = smg.nstokes < g->npol ? smg.nstokes : g->npol;
@@ -1386,7 +1388,7 @@ extern "C" int ddf_plan_grid(ddf_plan* P, float* vis, int32_t dopsf, void* strea
   if (p.nitems_a && batched_grid_a(p, v, P->place_g, P->V, P->queues, P->nwg, st)) return fail(DDF_EHIP, "grid pass A launch failed");
   P->mark(DDF_PASS_GRID_A, 1, st);
   P->mark(DDF_PASS_GRID_B, 0, st);
-  if (p.nitems_b && batched_grid_b(p, P->bplace_g, P->V, P->items_g, P->recs_g, P->queues + 1, P->nwg_tiles[0], P->slots_g,
+  if (p.nitems_b && batched_grid_b(p, P->bstamp_g, P->V, P->items_g, P->recs_g, P->queues + 1, P->nwg_tiles[0], P->slots_g,
                                      P->tile_first_g, P->tile_count_g, st))
     return fail(DDF_EHIP, "grid pass B launch failed");
   P->mark(DDF_PASS_GRID_B, 1, st);
@@ -1401,7 +1403,7 @@ extern "C" int ddf_plan_degrid(ddf_plan* P, float* vis, void* stream) {
   if (prepare_jones(P, st)) return fail(DDF_EHIP, "Jones preparation failed");
   if (batched_reset(P->queues + 2, 1, st)) return fail(DDF_EHIP, "launch failed");
   P->mark(DDF_PASS_DEGRID_A, 0, st);
-  if (P->pd.nitems_b && batched_degrid_a(P->pd, P->bplace_d, P->M, P->items_d, P->recs_d, P->queues + 2, P->nwg_tiles[1], st))
+  if (P->pd.nitems_b && batched_degrid_a(P->pd, P->bstamp_d, P->M, P->items_d, P->recs_d, P->queues + 2, P->nwg_tiles[1], st))
     return fail(DDF_EHIP, "degrid pass A launch failed");
   P->mark(DDF_PASS_DEGRID_A, 1, st);
   P->mark(DDF_PASS_DEGRID_B, 0, st);
@@ -1422,7 +1424,7 @@ extern "C" int ddf_plan_degrid_adjoint(ddf_plan* P, const float* vis_bar, void* 
     return fail(DDF_EHIP, "degrid adjoint pass B launch failed");
   P->mark(DDF_PASS_ADJ_B, 1, st);
   P->mark(DDF_PASS_ADJ_A, 0, st);
-  if (P->pd.nitems_b && batched_degrid_a_adjoint(P->pd, P->bplace_d, P->M, P->items_d, P->recs_d, P->queues + 3, P->nwg_tiles[2], st))
+  if (P->pd.nitems_b && batched_degrid_a_adjoint(P->pd, P->bstamp_d, P->M, P->items_d, P->recs_d, P->queues + 3, P->nwg_tiles[2], st))
     return fail(DDF_EHIP, "degrid adjoint pass A launch failed");
   P->mark(DDF_PASS_ADJ_A, 1, st);
   return DDF_OK;

@@ -174,7 +174,7 @@ int ddf_accumulate_weights(double* grid, int64_t grid_len, const float* weights,
 typedef struct ddf_plan ddf_plan;
 
 typedef struct ddf_plan_opts {
-  int32_t tile;            /* uv tile edge in grid cells (0: 32)                                */
+  int32_t tile;            /* uv tile edge in grid cells (0: 16)                                */
   int32_t chunk;           /* max records per tile work item (0: 2048)                          */
   int32_t blocks_per_item; /* BDA blocks per gridding work item (0: 128)                        */
   int32_t group;           /* facets per lane group in gridding (0: auto, <= 64)                */
@@ -215,6 +215,29 @@ int ddf_plan_info(const ddf_plan* plan, int64_t* out, int32_t n);
 int ddf_plan_timing(ddf_plan* plan, int32_t enable);
 int ddf_plan_pass_ms(ddf_plan* plan, float* ms, int32_t n);
 int ddf_plan_destroy(ddf_plan* plan);
+
+/* ---- on-device CF tables ----------------------------------------------------------------------
+ * Replaces the w-plane loop of ModCF.ClassWTermModified (DDFacet/Imager/ModCF.py:444-597) for
+ * many facets at once: per facet f and plane i of support S_i the S x S screen
+ * exp(-2 pi i (wlist_i / ref_wave) n1_f(l, m)) |taper_i|, zero-padded to (S_i overs)^2, centred
+ * FFT / N^2 in FP64, complex64, reorganised to [overs][overs][S][S]; the conjugate planes from
+ * the conjugate screen.  The host supplies what ModCF computes once per facet or support:
+ * coefs[f][36] the order-5 polyval2d coefficients of n(l, m) - n0 with the linear terms zeroed
+ * (ModCF.Give_dn, ModCF.py:267-295), tapers the |SpheMachine.MakeSphe(S_i)| of every plane
+ * concatenated ([nw][S_i^2]), lrad the facet half-width in radians.
+ * out (device, complex64): nfacets blocks of the 2*nw planes in put() order, back to back;
+ * plane_offset (host, [2*nw]) receives each plane's offset in a facet block (in complex
+ * elements), the same for every facet.                                                          */
+typedef struct ddf_cf_spec {
+  int32_t nw, overs, nfacets, pad_;
+  const int32_t* sups;   /* host [nw], odd                                                      */
+  const double* wlist;   /* host [nw], linspace(0, wmax, nw)                                     */
+  const double* coefs;   /* host [nfacets][36]                                                   */
+  const double* tapers;  /* host [sum of S_i^2]                                                  */
+  double ref_wave, lrad;
+} ddf_cf_spec;
+
+int ddf_cf_build(const ddf_cf_spec* spec, float* out, int64_t* plane_offset, void* stream);
 
 /* ---- facet image side of the major cycle ---------------------------------------------------
  * One facet of the image plane.  Device pointers; n = npix_facet (the padded facet size).      */

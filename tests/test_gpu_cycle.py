@@ -11,10 +11,10 @@ from conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def _cycle(jones=False, bda="pervis", precision="mixed"):
+def _cycle(jones=False, bda="pervis", precision="mixed", nband=1):
     from ddfacet_amd import configs
     from ddfacet_amd.cycle import MajorCycle
-    P, geom = configs.small_cycle(jones=jones, bda=bda)
+    P, geom = configs.small_cycle(jones=jones, bda=bda, nband=nband)
     return P, geom, MajorCycle(P, geom, precision=precision, tile=16)
 
 
@@ -72,14 +72,16 @@ def test_dirty_image_vs_fp64_dft():
     assert err < 5e-2
 
 
-def test_predict_matches_oracle_pipeline():
+@pytest.mark.parametrize("nband", [1, 2])
+def test_predict_matches_oracle_pipeline(nband):
     from oracle import cpuref
     from oracle import imaging as ref
-    P, geom, cyc = _cycle(jones=True)
+    P, geom, cyc = _cycle(jones=True, nband=nband)
     rng = np.random.default_rng(3)
-    model = np.zeros((1, 1, geom.m, geom.m), np.float32)
+    model = np.zeros((nband, 1, geom.m, geom.m), np.float32)
     X, Y = P.extra["src_pix"]
-    model[0, 0, X, Y] = rng.uniform(0.5, 1.5, X.size).astype(np.float32)
+    for b in range(nband):
+        model[b, 0, X, Y] = rng.uniform(0.5, 1.5, X.size).astype(np.float32)
     vis = torch.from_numpy(P.vis).cuda()
     cyc.predict(torch.from_numpy(model).cuda(), vis=vis)
     got = P.vis - vis.cpu().numpy()
@@ -110,12 +112,13 @@ def test_predict_vs_fp64_dft():
     assert err < 5e-2
 
 
-def test_autograd_dot_product_adjoint():
+@pytest.mark.parametrize("nband", [1, 2])
+def test_autograd_dot_product_adjoint(nband):
     from ddfacet_amd.autograd import DegridOperator
-    P, geom, cyc = _cycle(jones=True)
+    P, geom, cyc = _cycle(jones=True, nband=nband)
     op = DegridOperator(cyc)
     rng = np.random.default_rng(11)
-    x = torch.from_numpy(rng.standard_normal((1, 1, geom.m, geom.m)).astype(np.float32)).cuda()
+    x = torch.from_numpy(rng.standard_normal((nband, 1, geom.m, geom.m)).astype(np.float32)).cuda()
     y = torch.from_numpy((rng.standard_normal(P.vis.shape) + 1j * rng.standard_normal(P.vis.shape)).astype(np.complex64)).cuda()
     ax = op.forward(x)
     aty = op.adjoint(y)
