@@ -16,14 +16,15 @@
 //   grid B  persistent over bins: the S x S stamps of a bin accumulate in an LDS tile (T + 2 h)^2,
 //           flushed once into the facet grid with global float atomics (non-zero cells only).
 //   degrid A persistent over bins: the model tile is staged in LDS; one wavefront gathers each
-//           record's stamp (FP32 products summed in FP64 with a DPP wave reduction,
-//           degridder.h:202-212) -> M[b][f].
+//           record's stamp (FP32 products summed with a DPP wave reduction, in FP64 as
+//           degridder.h:202-212 in the FP64 precision, in FP32 otherwise) -> M[b][f].
 //   degrid B lanes are visibilities; each lane sums the model of every facet (Stokes -> corr,
 //           Jones, phasor, decorrelation) in registers and writes its visibility once.
 // Work queues are device counters reset by a small kernel (no memset nodes, capture-safe).
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -359,31 +360,34 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
         // Jones channel), so J0^H (sum_v meas_v w_v) J1 over a run of one key equals the sum of
         // the per-visibility products (gridder.h:255-260): the block sum is accumulated per run
         // and the Jones pair applied once per run (rows and channels in the reference order).
-        cd racc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        // run sums and the run's Jones pair in the precision's arithmetic type (FP32 outside the
+        // FP64 precision); the weight and Jones-weight sums stay FP64
+        using RT = typename std::conditional<PREC == 1, double, float>::type;
+        cx<RT> racc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
         double rfw = 0;  // PSF: sum of the run's weights
         int kt = -1, kc = -1, bt = -1, bc = -1;
-        JonesPair cur;
-        cur.BB = 0;
+        M2T<RT> cJ0, cJ1;
+        double cBB = 0;
         auto flush = [&]() {
-          M2 m;
+          M2T<RT> m;
           if (p.dopsf) {
-            m = mul(mul(herm(cur.J0), mul(cur.J0, herm(cur.J1))), cur.J1);
+            m = mul(mul(herm(cJ0), mul(cJ0, herm(cJ1))), cJ1);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) m.a[q] = m.a[q] * (dec * rfw);
+            for (int q = 0; q < 4; ++q) m.a[q] = m.a[q] * RT(dec * rfw);
           } else {
-            m = mul(mul(herm(cur.J0), M2{{racc[0], racc[1], racc[2], racc[3]}}), cur.J1);
+            m = mul(mul(herm(cJ0), M2T<RT>{{racc[0], racc[1], racc[2], racc[3]}}), cJ1);
           }
           if (READ == 4) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q] = acc[q] + m.a[q];
+            for (int q = 0; q < 4; ++q) acc[q] = acc[q] + cvt<double>(m.a[q]);
           } else if (READ == 2) {
-            acc[0] = acc[0] + m.a[0];
-            acc[3] = acc[3] + m.a[3];
+            acc[0] = acc[0] + cvt<double>(m.a[0]);
+            acc[3] = acc[3] + cvt<double>(m.a[3]);
           } else {
-            acc[0] = acc[0] + m.a[0];
+            acc[0] = acc[0] + cvt<double>(m.a[0]);
           }
 #pragma unroll
-          for (int q = 0; q < 4; ++q) racc[q] = cd{0, 0};
+          for (int q = 0; q < 4; ++q) racc[q] = cx<RT>{0, 0};
           rfw = 0;
         };
         bool any = false;
@@ -398,34 +402,44 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
           const int b_t = jp.beam ? jp.btime[row] : -1;
           for (int c = br.ch0; c < br.ch1; ++c) {
             const size_t vc = size_t(row) * v.nchan + c;
-            if (gptr(v.flags)[vc * v.ncorr]) continue;
+            // the visibility's flag, weight and values are loaded together (the values of a
+            // flagged visibility are valid memory, read and discarded)
+            const uint8_t flag = gptr(v.flags)[vc * v.ncorr];
+            const double fw = double(gptr(v.weights)[vc]);
+            const DDF_GLOBAL float2* vp = gptr(v.vis) + vc * v.ncorr;
+            cx<RT> meas[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              meas[q] = (q < READ) ? cx<RT>{RT(vp[q].x), RT(vp[q].y)} : cx<RT>{0, 0};
+            if (flag) continue;
             const int k_c = jp.kms ? jp.kchan[c] : -1, b_c = jp.beam ? jp.bchan[c] : -1;
             if (!any || k_t != kt || k_c != kc || b_t != bt || b_c != bc) {
               if (any) flush();
-              cur = jones_at<true>(jp, row, c, uvw, p.wl_mean);
+              const JonesPair jj = jones_at<true>(jp, row, c, uvw, p.wl_mean);
+              cJ0 = cvt<RT>(jj.J0);
+              cJ1 = cvt<RT>(jj.J1);
+              cBB = jj.BB;
               kt = k_t; kc = k_c; bt = b_t; bc = b_c;
               any = true;
             }
-            const double fw = double(gptr(v.weights)[vc]);
             const double fwd = fw * dec * dec;
             if (p.dopsf) {
               rfw += fw;
             } else {
-              const cd wt = ph.at(v, c) * fw;
-              const DDF_GLOBAL float2* vp = gptr(v.vis) + vc * v.ncorr;
+              const cx<RT> wt = cvt<RT>(ph.at(v, c) * fw);
               if (READ == 4) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) racc[q] = racc[q] + cd{vp[q].x, vp[q].y} * wt;
+                for (int q = 0; q < 4; ++q) racc[q] = racc[q] + meas[q] * wt;
               } else if (READ == 2) {
-                racc[0] = racc[0] + cd{vp[0].x, vp[0].y} * wt;
-                racc[3] = racc[3] + cd{vp[1].x, vp[1].y} * wt;
+                racc[0] = racc[0] + meas[0] * wt;
+                racc[3] = racc[3] + meas[1] * wt;
               } else {
-                const cd x = cd{vp[0].x, vp[0].y} * wt;
+                const cx<RT> x = meas[0] * wt;
                 racc[0] = racc[0] + x;
                 racc[3] = racc[3] + x;
               }
             }
-            const double bbw = cur.BB * fwd;
+            const double bbw = cBB * fwd;
             sj += bbw;
             const double x0 = double(npg) * bbw, x1 = double(npg) * fwd;
             if (p.chan_lds) {
@@ -585,6 +599,21 @@ __device__ __forceinline__ double wave_total(double x) {
   return __longlong_as_double((long long)(uint64_t(lane_u32(uint32_t(b), 63)) | (uint64_t(lane_u32(uint32_t(b >> 32), 63)) << 32)));
 }
 
+// FP32 counterpart of wave_total (same DPP sequence, one 32-bit move per step).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_f32(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROWMASK, 0xF, true));
+}
+__device__ __forceinline__ float wave_total_f32(float x) {
+  x += dpp_f32<0xB1, 0xF>(x);
+  x += dpp_f32<0x4E, 0xF>(x);
+  x += dpp_f32<0x124, 0xF>(x);
+  x += dpp_f32<0x128, 0xF>(x);
+  x += dpp_f32<0x142, 0xA>(x);
+  x += dpp_f32<0x143, 0xC>(x);
+  return lane_f32(x, 63);
+}
+
 // Stamp record of a binned (block, facet) record (plan_stamp_records): base | S << 16 | cfoff << 32,
 // with base the tile offset of the stamp's first cell (row (locy - sup) - y0, column
 // (locx - sup) - x0 of the (T + 2 h)^2 tile image) and cfoff the element offset of the record's
@@ -680,10 +709,10 @@ __global__ __launch_bounds__(64) void grid_pass_b(BatchParams p, const uint64_t*
       const uint32_t k = k0 + lane;
       const bool ok = k < bi.end;
       const uint64_t my_rec = ok ? bstamp[k] : 0;
-      const uint32_t my_t = ok ? recs[k] : 0;
+      const size_t my_v0 = ok ? size_t(recs[k]) * p.nstokes : 0;
       float2 my_v[4];
 #pragma unroll
-      for (int ip = 0; ip < 4; ++ip) my_v[ip] = (ok && ip < npg) ? V[size_t(my_t) * p.nstokes + ip] : make_float2(0.f, 0.f);
+      for (int ip = 0; ip < 4; ++ip) my_v[ip] = (ok && ip < npg) ? V[my_v0 + ip] : make_float2(0.f, 0.f);
       const int cnt = int(min(64u, bi.end - k0));
       Stamp cur = stamp_of(lane_u64(my_rec, 0), cf);
       StampCells cc;
@@ -790,8 +819,9 @@ __global__ __launch_bounds__(256) void grid_slot_reduce(BatchParams p, const flo
 // stamp is gathered by one wavefront (next record's CF values loaded ahead): FP32 products
 // (grid x CF) summed in FP64, per lane and then across the wavefront (degridder.h:202-212 sums
 // the FP32 products in double).
+template <typename ACC>
 __device__ __forceinline__ void stamp_gather(const float* tile, int WW, int ns, const Stamp& s, const StampCells& sc,
-                                             double* sr, double* si) {
+                                             ACC* sr, ACC* si) {
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     if (sc.off[r] < 0) continue;
@@ -800,13 +830,19 @@ __device__ __forceinline__ void stamp_gather(const float* tile, int WW, int ns, 
     for (int ip = 0; ip < 4; ++ip) {
       if (ip < ns) {
         const float mr = t[(2 * ip) * WW], mi = t[(2 * ip + 1) * WW];
-        sr[ip] += double(mr * sc.w[r].x - mi * sc.w[r].y);
-        si[ip] += double(mr * sc.w[r].y + mi * sc.w[r].x);
+        sr[ip] += ACC(mr * sc.w[r].x - mi * sc.w[r].y);
+        si[ip] += ACC(mr * sc.w[r].y + mi * sc.w[r].x);
       }
     }
   }
 }
+__device__ __forceinline__ double wave_sum_acc(double x) { return wave_total(x); }
+__device__ __forceinline__ float wave_sum_acc(float x) { return wave_total_f32(x); }
 
+// ACC: the accumulator of the stamp sums (double as the reference; float in the mixed and FP32
+// precisions, where the FP32 products of one stamp are summed in FP32 per lane and across the
+// wavefront)
+template <typename ACC>
 __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64_t* bstamp, float2* M,
                                                      const BinItem* items, const uint32_t* recs, uint32_t* queue) {
   extern __shared__ float tile[];  // [nstokes][2][W * W]
@@ -855,7 +891,7 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
           nxt = stamp_of(lane_u64(my_rec, j + 1), cf);
           fetch_cells(nxt, lane, W, nc);
         }
-        double sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
+        ACC sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
         stamp_gather(tile, WW, ns, cur, cc, sr, si);
         for (int c0 = 256; c0 < cur.S2; c0 += 256) {
           StampCells xc;
@@ -870,7 +906,7 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
 #pragma unroll
         for (int ip = 0; ip < 4; ++ip) {
           if (ip < ns) {
-            const double a = wave_total(sr[ip]), b = wave_total(si[ip]);
+            const ACC a = wave_sum_acc(sr[ip]), b = wave_sum_acc(si[ip]);
             if (lane == j) res[ip] = make_float2(float(a), float(b));
           }
         }
@@ -891,6 +927,7 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
 template <int NCORR, int JMODE, int PREC>
 __global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v, const float2* M,
                                                      const uint64_t* vrc, const uint32_t* vblk, int64_t nvl) {
+  using RT = typename std::conditional<PREC == 1, double, float>::type;
   for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < nvl; k += int64_t(gridDim.x) * blockDim.x) {
     const uint64_t e = vrc[k];
     const int64_t row = int64_t(uint32_t(e));
@@ -920,12 +957,15 @@ __global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v
       }
       if (!any) continue;  // dropped record, or a zero model: contributes exactly 0
       const FacetDev fd = gload(p.facets + f);
-      M2 cv;
+      // Stokes -> correlations and the Jones products in the precision's arithmetic type (FP32
+      // outside the FP64 precision); the sum over facets stays FP64
+      M2T<RT> cv;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        cv.a[q] = cd{0, 0};
+        cv.a[q] = cx<RT>{0, 0};
 #pragma unroll
-        for (int s = 0; s < 4; ++s) cv.a[q] = cv.a[q] + cd{p.sm[q][s][0], p.sm[q][s][1]} * st[s];
+        for (int s = 0; s < 4; ++s)
+          if (s < p.nstokes) cv.a[q] = cv.a[q] + cx<RT>{RT(p.sm[q][s][0]), RT(p.sm[q][s][1])} * cvt<RT>(st[s]);
       }
       JonesP jp = p.jp;
       jp.kdir = fd.kms_dir;
@@ -939,8 +979,8 @@ __global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v
       if (JMODE == 2) {
         const int64_t mid = br.rows[br.nr / 2];
         if (mid >= 0 && mid < v.nrow) {
-          JonesPair jj = jones_at<true>(jp, mid, (br.ch0 + br.ch1) / 2, v.uvw + 3 * mid, p.wl_mean);
-          cv = mul(mul(jj.J0, cv), herm(jj.J1));
+          const JonesPair jj = jones_at<true>(jp, mid, (br.ch0 + br.ch1) / 2, v.uvw + 3 * mid, p.wl_mean);
+          cv = mul(mul(cvt<RT>(jj.J0), cv), herm(cvt<RT>(jj.J1)));
         }
       }
       double dec = 1.;
@@ -952,13 +992,13 @@ __global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v
         if (mid >= 0 && mid < v.nrow) dec = decorr_factor(dp, v.uvw, fm, mid);
       }
       const double angle = 2. * kPi * (guvw[0] * fd.l0 + guvw[1] * fd.m0 + guvw[2] * fd.n0) / kC;
-      const cd corr = phasor_prec<PREC>(fr, angle) * dec;
+      const cx<RT> corr = cvt<RT>(phasor_prec<PREC>(fr, angle) * dec);
       if (JMODE == 1) {
-        JonesPair jj = jones_at<true>(jp, row, c, uvw, p.wl_mean);
-        cv = mul(mul(jj.J0, cv), herm(jj.J1));
+        const JonesPair jj = jones_at<true>(jp, row, c, uvw, p.wl_mean);
+        cv = mul(mul(cvt<RT>(jj.J0), cv), herm(cvt<RT>(jj.J1)));
       }
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[q] = acc[q] + cv.a[q] * corr;
+      for (int q = 0; q < 4; ++q) acc[q] = acc[q] + cvt<double>(cv.a[q] * corr);
     }
     DDF_GLOBAL float2* vp = gptr_w(v.vis) + (size_t(row) * v.nchan + c) * NCORR;
     if (NCORR == 4) {
@@ -1170,7 +1210,8 @@ int batched_tile_wgs(const BatchParams& pg, const BatchParams& pd, unsigned out[
   const int Wg = pg.tile + 2 * pg.halo, Wd = pd.tile + 2 * pd.halo;
   const int npg = pg.nstokes < pg.npol ? pg.nstokes : pg.npol;
   out[0] = persistent_wgs(grid_pass_b<false>, size_t(npg) * Wg * Wg * sizeof(float2), 0, 64);
-  out[1] = persistent_wgs(degrid_pass_a, size_t(pd.nstokes) * Wd * Wd * 2 * sizeof(float), 0);
+  out[1] = pd.prec == 1 ? persistent_wgs(degrid_pass_a<double>, size_t(pd.nstokes) * Wd * Wd * 2 * sizeof(float), 0)
+                        : persistent_wgs(degrid_pass_a<float>, size_t(pd.nstokes) * Wd * Wd * 2 * sizeof(float), 0);
   out[2] = persistent_wgs(grid_pass_b<true>, size_t(pd.nstokes) * Wd * Wd * sizeof(float2), 0, 64);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
@@ -1211,8 +1252,13 @@ int batched_degrid_a(const BatchParams& p, const uint64_t* place, float2* M, con
                      const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st) {
   const int W = p.tile + 2 * p.halo;
   size_t lds = size_t(p.nstokes) * W * W * 2 * sizeof(float);
-  nwg = persistent_wgs(degrid_pass_a, lds, nwg);
-  hipLaunchKernelGGL(degrid_pass_a, dim3(nwg), dim3(256), lds, st, p, place, M, items, recs, queue);
+  if (p.prec == 1) {
+    nwg = persistent_wgs(degrid_pass_a<double>, lds, nwg);
+    hipLaunchKernelGGL(degrid_pass_a<double>, dim3(nwg), dim3(256), lds, st, p, place, M, items, recs, queue);
+  } else {
+    nwg = persistent_wgs(degrid_pass_a<float>, lds, nwg);
+    hipLaunchKernelGGL(degrid_pass_a<float>, dim3(nwg), dim3(256), lds, st, p, place, M, items, recs, queue);
+  }
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 

@@ -58,29 +58,49 @@ constexpr double kPi = 3.141592653589793238462643383279502884197;
 constexpr int kWave = 64;
 
 // ---- complex -------------------------------------------------------------------------------
-struct cd {
-  double re, im;
+// cx<T>: complex in FP64 (cd, the reference's dcmplx) or FP32 (cf); M2T<T>: 2 x 2 complex matrix
+// (DDF::Mat, JonesServer.h:38-83) with M2 = M2T<double>.
+template <typename T>
+struct cx {
+  T re, im;
 };
+using cd = cx<double>;
+using cf = cx<float>;
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return cx<T>{a.re + b.re, a.im + b.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return cx<T>{a.re - b.re, a.im - b.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator*(cx<T> a, cx<T> b) {
+  return cx<T>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> operator*(cx<T> a, T s) { return cx<T>{a.re * s, a.im * s}; }
+template <typename T>
+__device__ __forceinline__ cx<T> conj(cx<T> a) { return cx<T>{a.re, -a.im}; }
 __device__ __forceinline__ cd mk(double r, double i) { return cd{r, i}; }
-__device__ __forceinline__ cd operator+(cd a, cd b) { return cd{a.re + b.re, a.im + b.im}; }
-__device__ __forceinline__ cd operator-(cd a, cd b) { return cd{a.re - b.re, a.im - b.im}; }
-__device__ __forceinline__ cd operator*(cd a, cd b) {
-  return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
-}
-__device__ __forceinline__ cd operator*(cd a, double s) { return cd{a.re * s, a.im * s}; }
-__device__ __forceinline__ cd conj(cd a) { return cd{a.re, -a.im}; }
 __device__ __forceinline__ double cabs(cd a) { return hypot(a.re, a.im); }
+template <typename T, typename U>
+__device__ __forceinline__ cx<T> cvt(cx<U> a) { return cx<T>{T(a.re), T(a.im)}; }
 
-struct M2 {
-  cd a[4];
+template <typename T>
+struct M2T {
+  cx<T> a[4];
 };
+using M2 = M2T<double>;
 __device__ __forceinline__ M2 unity() { return M2{{cd{1, 0}, cd{0, 0}, cd{0, 0}, cd{1, 0}}}; }
-__device__ __forceinline__ M2 mul(const M2& x, const M2& y) {
-  return M2{{x.a[0] * y.a[0] + x.a[1] * y.a[2], x.a[0] * y.a[1] + x.a[1] * y.a[3],
-             x.a[2] * y.a[0] + x.a[3] * y.a[2], x.a[2] * y.a[1] + x.a[3] * y.a[3]}};
+template <typename T>
+__device__ __forceinline__ M2T<T> mul(const M2T<T>& x, const M2T<T>& y) {
+  return M2T<T>{{x.a[0] * y.a[0] + x.a[1] * y.a[2], x.a[0] * y.a[1] + x.a[1] * y.a[3],
+                 x.a[2] * y.a[0] + x.a[3] * y.a[2], x.a[2] * y.a[1] + x.a[3] * y.a[3]}};
 }
-__device__ __forceinline__ M2 herm(const M2& x) {
-  return M2{{conj(x.a[0]), conj(x.a[2]), conj(x.a[1]), conj(x.a[3])}};
+template <typename T>
+__device__ __forceinline__ M2T<T> herm(const M2T<T>& x) {
+  return M2T<T>{{conj(x.a[0]), conj(x.a[2]), conj(x.a[1]), conj(x.a[3])}};
+}
+template <typename T, typename U>
+__device__ __forceinline__ M2T<T> cvt(const M2T<U>& x) {
+  return M2T<T>{{cvt<T>(x.a[0]), cvt<T>(x.a[1]), cvt<T>(x.a[2]), cvt<T>(x.a[3])}};
 }
 
 // ---- wavefront reductions (all lanes receive the total) ------------------------------------

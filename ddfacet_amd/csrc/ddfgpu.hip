@@ -1433,6 +1433,7 @@ extern "C" int ddf_plan_degrid_adjoint(ddf_plan* P, const float* vis_bar, void* 
 extern "C" int ddf_plan_info(const ddf_plan* P, int64_t* out, int32_t n) {
   if (!P || !out) return fail(DDF_EINVAL, "null argument");
   for (int i = 0; i < n && i < 8; ++i) out[i] = P->stats[i];
+  for (int i = 8; i < n && i < 11; ++i) out[i] = int64_t(P->nwg_tiles[i - 8]);
   return DDF_OK;
 }
 

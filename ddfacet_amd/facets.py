@@ -211,10 +211,10 @@ class DeviceFacetSet:
         return self.plan_info()
 
     def plan_info(self):
-        a = (C.c_int64 * 8)()
-        check(lib().ddf_plan_info(self._plan, a, 8))
+        a = (C.c_int64 * 11)()
+        check(lib().ddf_plan_info(self._plan, a, 11))
         keys = ["grid_records", "degrid_records", "grid_items", "degrid_items", "degrid_vis", "device_bytes",
-                "tile", "facet_group"]
+                "tile", "facet_group", "grid_tile_workgroups", "degrid_tile_workgroups", "adjoint_tile_workgroups"]
         return dict(zip(keys, list(a)))
 
     def plan_timing(self, enable=True):

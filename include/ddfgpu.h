@@ -178,7 +178,11 @@ typedef struct ddf_plan_opts {
   int32_t chunk;           /* max records per tile work item (0: 2048)                          */
   int32_t blocks_per_item; /* BDA blocks per gridding work item (0: 128)                        */
   int32_t group;           /* facets per lane group in gridding (0: auto, <= 64)                */
-  int32_t precision;       /* 0 FP64 phase base + FP32 sincos + FP64 block sums; 1 FP64; 2 FP32 */
+  int32_t precision;       /* 0 mixed: FP64 uvw rotation and per-row phase base, FP32 sincos and
+                              FP32 phasor advance along channels (re-anchored every 16), FP32
+                              block sums / Jones / stamp sums, FP64 weight sums;
+                              1 FP64 arithmetic throughout (as the reference);
+                              2 FP32 throughout (phase formed and reduced in FP32)              */
   int32_t workgroups;      /* persistent workgroups per launch (0: 4 per compute unit for the
                               per-block passes, as many as fit on the device for the tile passes) */
 } ddf_plan_opts;
@@ -199,7 +203,8 @@ int ddf_plan_degrid(ddf_plan* plan, float* vis, void* stream);
  * the grids first; vis_bar is complex64 [nrow][nchan][ncorr].                                   */
 int ddf_plan_degrid_adjoint(ddf_plan* plan, const float* vis_bar, void* stream);
 /* Counters: [0] grid records, [1] degrid records, [2] grid work items, [3] degrid work items,
- * [4] degrid visibilities, [5] device bytes held, [6] tile, [7] facet group.                    */
+ * [4] degrid visibilities, [5] device bytes held, [6] tile, [7] facet group, [8]-[10] persistent
+ * workgroups of the grid tile pass, the degrid tile pass and the adjoint tile pass.            */
 int ddf_plan_info(const ddf_plan* plan, int64_t* out, int32_t n);
 /* Per-pass timing, a measurement hook with no reference counterpart: with timing on, every pass
  * of ddf_plan_grid / ddf_plan_degrid / ddf_plan_degrid_adjoint is bracketed by HIP events on the
