@@ -202,7 +202,8 @@ __global__ __launch_bounds__(256) void plan_stamp_records(BatchParams p, const u
 // [t][dir][ant][chan][4] through GiveJones (interpolation mode) and, for the DD table, the
 // amplitude / phase part of NormJones, in FP64 at [t][ant][chan][dir][4] (facet-fastest: the
 // lanes of a wavefront are facets).  ScaleJones depends on the visibility and stays in the passes.
-__global__ __launch_bounds__(256) void plan_prepare_jones(const float2* src, double2* dst, JonesP jp, int dd) {
+__global__ __launch_bounds__(256) void plan_prepare_jones(const float2* src, double2* dst, float2* dstf, JonesP jp,
+                                                          int dd) {
   const int* dims = dd ? jp.kdims : jp.bdims;
   const int nt = dims[0], nd = dims[1], na = dims[2], nch = dims[3];
   JonesP q = jp;
@@ -218,7 +219,10 @@ __global__ __launch_bounds__(256) void plan_prepare_jones(const float2* src, dou
     if (dd) norm_jones(q, J, nullptr, 0.);
     const int64_t o = ((int64_t(t) * na + a) * nch + ch) * nd + d;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) dst[o * 4 + k] = make_double2(J.a[k].re, J.a[k].im);
+    for (int k = 0; k < 4; ++k) {
+      dst[o * 4 + k] = make_double2(J.a[k].re, J.a[k].im);
+      dstf[o * 4 + k] = make_float2(float(J.a[k].re), float(J.a[k].im));
+    }
   }
 }
 
@@ -262,13 +266,19 @@ __global__ void reset_counter(uint32_t* c, int n) {
 template <int READ, int JMODE, int PREC>
 __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, const uint64_t* place, float2* V,
                                                    uint32_t* queue) {
-  // LDS: per-facet sums of every facet [nf][nband*npol + 2*nband], flushed once at the end, and
-  // the per-channel Jones sums of the current facet group [G][2][nchan] (when they fit),
-  // flushed when the workgroup moves to another group.  Work items are group-major, so a
-  // workgroup changes group at most a few times.
+  // LDS: per-facet sums of every facet [nf][nband*npol + 2*nband], flushed once at the end; the
+  // per-channel Jones sums of the current facet group [G][2][nchan] (when they fit), flushed when
+  // the workgroup moves to another group (work items are group-major, so a workgroup changes
+  // group at most a few times); and one staging buffer per slot.
+  //
+  // Slots are aligned to wavefronts (64 / G slots of G lanes per wavefront): the lanes of a slot
+  // are the facets of the group for one block; they stage the block's visibilities (values,
+  // weight, flag) in LDS, G at a time, one per lane, and then every lane walks the staged
+  // visibilities in the reference order.  A wavefront's LDS operations complete in order,
+  // so the staging needs no workgroup barrier.
   extern __shared__ double lds[];
   const int G = p.group;
-  const int slots = p.slots;
+  const int spw = 64 / G;
   const int nbp = p.nband * p.npol;
   const int per_f = nbp + 2 * p.nband;
   double* s_acc = lds;
@@ -277,11 +287,18 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
   // lanes (facets) of a wavefront fall on different LDS banks
   const int cs = 2 * v.nchan + 1;
   const int nchan_acc = p.chan_lds ? G * cs : 0;
+  constexpr int kStride = 2 * READ + 2;  // staged entry: READ complex values, weight, flag
+  float* s_stage = reinterpret_cast<float*>(lds + p.nf * per_f + nchan_acc);
   for (int i = threadIdx.x; i < p.nf * per_f + nchan_acc; i += blockDim.x) lds[i] = 0.;
   __syncthreads();
-  const int slot = threadIdx.x / G;
-  const int lf = threadIdx.x - slot * G;
-  const bool active = slot < slots;
+  const int wl = threadIdx.x & 63;
+  const int sw = wl / G;
+  const int slot = (threadIdx.x >> 6) * spw + sw;
+  const int slots = (blockDim.x >> 6) * spw;
+  const int lf = wl - sw * G;
+  const bool active = sw < spw;
+  const int K = G;  // visibilities staged per slot and round
+  float* stg = s_stage + size_t(active ? slot : 0) * K * kStride;
   const int npg = p.nstokes < p.npol ? p.nstokes : p.npol;
   __shared__ uint32_t s_item;
   int cur_group = -1;
@@ -312,20 +329,22 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
       cur_group = group;
     }
     const int f = group * G + lf;
-    const bool fok = active && f < p.nf;
+    const int fc = f < p.nf ? f : p.nf - 1;
     const int64_t b0 = chunk * p.blocks_per_item;
     const int64_t b1 = min(b0 + p.blocks_per_item, v.nblocks);
-    for (int64_t b = b0 + slot; fok && b < b1; b += slots) {
-      const uint64_t rec = place[b * p.nf + f];
-      if (!place_valid(rec)) continue;
-      const FacetDev fd = gload(p.facets + f);
+    for (int64_t b = b0 + slot; active && b < b1; b += slots) {
+      // every lane of the slot stages; lanes whose record is dropped (or past the last facet)
+      // only stage
+      const uint64_t rec = place[b * p.nf + fc];
+      const bool doit = f < p.nf && place_valid(rec);
+      const FacetDev fd = gload(p.facets + fc);
       BlockRec br = read_block(v, b);
       const int64_t mid = br.rows[br.nr / 2];
       const bool mid_ok = mid >= 0 && mid < v.nrow;
       DecorrP dp = p.dp;
       dp.l0 = fd.dec_l0;
       dp.m0 = fd.dec_m0;
-      const double dec = mid_ok ? decorr_factor(dp, v.uvw, p.fmean0, mid) : 1.;
+      const double dec = (doit && mid_ok) ? decorr_factor(dp, v.uvw, p.fmean0, mid) : 1.;
       JonesP jp = p.jp;
       jp.kdir = fd.kms_dir;
       jp.bdir = fd.beam_dir;
@@ -334,9 +353,9 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
       const Unpacked u = unpack_place(rec);
       double BBblk = 0;
       JonesPair jblk;
-      if (JMODE == 2) {
+      if (JMODE == 2 && doit) {
         // block Jones at the block means (gridder.h:291-301), means in the reference order
-        double su = 0, sv = 0, sw = 0;
+        double su = 0, sv = 0, sw2 = 0;
         int cnt = 0;
         for (int r = 0; r < br.nr; ++r) {
           const int64_t row = br.rows[r];
@@ -346,186 +365,206 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
             if (gptr(v.flags)[(size_t(row) * v.nchan + c) * v.ncorr]) continue;
             su += shifted(q[0], q[2], fd.cu);
             sv += shifted(q[1], q[2], fd.cv);
-            sw += q[2];
+            sw2 += q[2];
             ++cnt;
           }
         }
-        double uvwm[3] = {su / cnt, sv / cnt, sw / cnt};
+        double uvwm[3] = {su / cnt, sv / cnt, sw2 / cnt};
         jblk = jones_at<true>(jp, mid_ok ? mid : 0, (br.ch0 + br.ch1) / 2, uvwm, p.wl_mean);
         BBblk = jblk.BB;
       }
       double* sch = s_chan + lf * cs;
-      if (JMODE == 1) {
-        // Per-visibility Jones without ScaleJones depends only on the key (baseline, Jones time,
-        // Jones channel), so J0^H (sum_v meas_v w_v) J1 over a run of one key equals the sum of
-        // the per-visibility products (gridder.h:255-260): the block sum is accumulated per run
-        // and the Jones pair applied once per run (rows and channels in the reference order).
-        // run sums and the run's Jones pair in the precision's arithmetic type (FP32 outside the
-        // FP64 precision); the weight and Jones-weight sums stay FP64
-        using RT = typename std::conditional<PREC == 1, double, float>::type;
-        cx<RT> racc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-        double rfw = 0;  // PSF: sum of the run's weights
-        int kt = -1, kc = -1, bt = -1, bc = -1;
-        M2T<RT> cJ0, cJ1;
-        double cBB = 0;
-        auto flush = [&]() {
-          M2T<RT> m;
-          if (p.dopsf) {
-            m = mul(mul(herm(cJ0), mul(cJ0, herm(cJ1))), cJ1);
+      using RT = typename std::conditional<PREC == 1, double, float>::type;
+      // Per-visibility Jones without ScaleJones (JMODE 1) depends only on the key (baseline,
+      // Jones time, Jones channel), so J0^H (sum_v meas_v w_v) J1 over a run of one key equals
+      // the sum of the per-visibility products (gridder.h:255-260): the run sum is accumulated
+      // in the precision's arithmetic type and the Jones pair applied once per run.
+      cx<RT> racc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+      double rfw = 0;  // PSF: sum of the run's weights
+      int kt = -1, kc = -1, bt = -1, bc = -1;
+      M2T<RT> cJ0, cJ1;
+      double cBB = 0;
+      bool any = false;
+      auto flush = [&]() {
+        M2T<RT> m;
+        if (p.dopsf) {
+          m = mul(mul(herm(cJ0), mul(cJ0, herm(cJ1))), cJ1);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) m.a[q] = m.a[q] * RT(dec * rfw);
-          } else {
-            m = mul(mul(herm(cJ0), M2T<RT>{{racc[0], racc[1], racc[2], racc[3]}}), cJ1);
+          for (int q = 0; q < 4; ++q) m.a[q] = m.a[q] * RT(dec * rfw);
+        } else {
+          m = mul(mul(herm(cJ0), M2T<RT>{{racc[0], racc[1], racc[2], racc[3]}}), cJ1);
+        }
+        if (READ == 4) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[q] = acc[q] + cvt<double>(m.a[q]);
+        } else if (READ == 2) {
+          acc[0] = acc[0] + cvt<double>(m.a[0]);
+          acc[3] = acc[3] + cvt<double>(m.a[3]);
+        } else {
+          acc[0] = acc[0] + cvt<double>(m.a[0]);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) racc[q] = cx<RT>{0, 0};
+        rfw = 0;
+      };
+      JonesCache<true> jc;
+      jc.reset();
+      // row state of the walk
+      int cur_r = -1;
+      int64_t row = -1;
+      bool row_ok = false;
+      const double* uvw = v.uvw;
+      int k_t = -1, b_t = -1;
+      ChanPhasor<PREC> ph;
+      const int nch = br.ch1 - br.ch0;
+      const int nvis = br.nr * nch;
+      for (int i0 = 0; i0 < nvis; i0 += K) {
+        const int n = min(K, nvis - i0);
+        // stage visibilities i0 .. i0 + n - 1 of the block (row-major)
+        for (int i = lf; i < n; i += G) {
+          const int g = i0 + i;
+          const int rr = g / nch;
+          const int cc = br.ch0 + (g - rr * nch);
+          const int64_t srow = br.rows[rr];
+          float* e = stg + i * kStride;
+          if (srow < 0 || srow >= v.nrow) {
+            e[2 * READ + 1] = 1.f;
+            continue;
           }
-          if (READ == 4) {
+          const size_t vc = size_t(srow) * v.nchan + cc;
+          const DDF_GLOBAL float2* vp = gptr(v.vis) + vc * v.ncorr;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q] = acc[q] + cvt<double>(m.a[q]);
-          } else if (READ == 2) {
-            acc[0] = acc[0] + cvt<double>(m.a[0]);
-            acc[3] = acc[3] + cvt<double>(m.a[3]);
-          } else {
-            acc[0] = acc[0] + cvt<double>(m.a[0]);
+          for (int q = 0; q < READ; ++q) {
+            e[2 * q] = vp[q].x;
+            e[2 * q + 1] = vp[q].y;
           }
-#pragma unroll
-          for (int q = 0; q < 4; ++q) racc[q] = cx<RT>{0, 0};
-          rfw = 0;
-        };
-        bool any = false;
-        for (int r = 0; r < br.nr; ++r) {
-          const int64_t row = br.rows[r];
-          if (row < 0 || row >= v.nrow) continue;
-          const double* uvw = v.uvw + 3 * row;
-        const DDF_GLOBAL double* guvw = gptr(v.uvw) + 3 * row;
-          ChanPhasor<PREC> ph;
-          ph.row(-2. * kPi * (guvw[0] * fd.l0 + guvw[1] * fd.m0 + guvw[2] * fd.n0) / kC);
-          const int k_t = jp.kms ? jp.ktime[row] : -1;
-          const int b_t = jp.beam ? jp.btime[row] : -1;
-          for (int c = br.ch0; c < br.ch1; ++c) {
-            const size_t vc = size_t(row) * v.nchan + c;
-            // the visibility's flag, weight and values are loaded together (the values of a
-            // flagged visibility are valid memory, read and discarded)
-            const uint8_t flag = gptr(v.flags)[vc * v.ncorr];
-            const double fw = double(gptr(v.weights)[vc]);
-            const DDF_GLOBAL float2* vp = gptr(v.vis) + vc * v.ncorr;
-            cx<RT> meas[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-              meas[q] = (q < READ) ? cx<RT>{RT(vp[q].x), RT(vp[q].y)} : cx<RT>{0, 0};
-            if (flag) continue;
-            const int k_c = jp.kms ? jp.kchan[c] : -1, b_c = jp.beam ? jp.bchan[c] : -1;
-            if (!any || k_t != kt || k_c != kc || b_t != bt || b_c != bc) {
-              if (any) flush();
-              const JonesPair jj = jones_at<true>(jp, row, c, uvw, p.wl_mean);
-              cJ0 = cvt<RT>(jj.J0);
-              cJ1 = cvt<RT>(jj.J1);
-              cBB = jj.BB;
-              kt = k_t; kc = k_c; bt = b_t; bc = b_c;
-              any = true;
-            }
-            const double fwd = fw * dec * dec;
-            if (p.dopsf) {
-              rfw += fw;
-            } else {
-              const cx<RT> wt = cvt<RT>(ph.at(v, c) * fw);
-              if (READ == 4) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) racc[q] = racc[q] + meas[q] * wt;
-              } else if (READ == 2) {
-                racc[0] = racc[0] + meas[0] * wt;
-                racc[3] = racc[3] + meas[1] * wt;
-              } else {
-                const cx<RT> x = meas[0] * wt;
-                racc[0] = racc[0] + x;
-                racc[3] = racc[3] + x;
+          e[2 * READ] = gptr(v.weights)[vc];
+          e[2 * READ + 1] = gptr(v.flags)[vc * v.ncorr] ? 1.f : 0.f;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (doit) {
+          int r = i0 / nch;
+          int c = br.ch0 + (i0 - r * nch);
+          for (int i = 0; i < n; ++i) {
+            if (r != cur_r) {
+              cur_r = r;
+              row = br.rows[r];
+              row_ok = row >= 0 && row < v.nrow;
+              if (row_ok) {
+                uvw = v.uvw + 3 * row;
+                const DDF_GLOBAL double* guvw = gptr(v.uvw) + 3 * row;
+                ph.row(-2. * kPi * (guvw[0] * fd.l0 + guvw[1] * fd.m0 + guvw[2] * fd.n0) / kC);
+                k_t = jp.kms ? jp.ktime[row] : -1;
+                b_t = jp.beam ? jp.btime[row] : -1;
               }
             }
-            const double bbw = cBB * fwd;
-            sj += bbw;
-            const double x0 = double(npg) * bbw, x1 = double(npg) * fwd;
-            if (p.chan_lds) {
-              atomicAdd(&sch[c], x0);
-              atomicAdd(&sch[v.nchan + c], x1);
+            const float* e = stg + i * kStride;
+            const int cv_ = c;
+            if (++c == br.ch1) {
+              c = br.ch0;
+              ++r;
+            }
+            if (!row_ok || e[2 * READ + 1] != 0.f) continue;
+            const double fw = double(e[2 * READ]);
+            const double fwd = fw * dec * dec;
+            if (JMODE == 1) {
+              const int k_c = jp.kms ? jp.kchan[cv_] : -1, b_c = jp.beam ? jp.bchan[cv_] : -1;
+              if (!any || k_t != kt || k_c != kc || b_t != bt || b_c != bc) {
+                if (any) flush();
+                const JonesPair jj = jones_at<true>(jp, row, cv_, uvw, p.wl_mean);
+                cJ0 = cvt<RT>(jj.J0);
+                cJ1 = cvt<RT>(jj.J1);
+                cBB = jj.BB;
+                kt = k_t; kc = k_c; bt = b_t; bc = b_c;
+                any = true;
+              }
+              if (p.dopsf) {
+                rfw += fw;
+              } else {
+                const cx<RT> wt = cvt<RT>(ph.at(v, cv_) * fw);
+                if (READ == 4) {
+#pragma unroll
+                  for (int q = 0; q < 4; ++q) racc[q] = racc[q] + cx<RT>{RT(e[2 * q]), RT(e[2 * q + 1])} * wt;
+                } else if (READ == 2) {
+                  racc[0] = racc[0] + cx<RT>{RT(e[0]), RT(e[1])} * wt;
+                  racc[3] = racc[3] + cx<RT>{RT(e[2]), RT(e[3])} * wt;
+                } else {
+                  const cx<RT> x = cx<RT>{RT(e[0]), RT(e[1])} * wt;
+                  racc[0] = racc[0] + x;
+                  racc[3] = racc[3] + x;
+                }
+              }
+              const double bbw = cBB * fwd;
+              sj += bbw;
+              const double x0 = double(npg) * bbw, x1 = double(npg) * fwd;
+              if (p.chan_lds) {
+                atomicAdd(&sch[cv_], x0);
+                atomicAdd(&sch[v.nchan + cv_], x1);
+              } else {
+                atomic_add_g(gptr_w(fd.sum_jones_chan) + cv_, x0);
+                atomic_add_g(gptr_w(fd.sum_jones_chan) + (v.nchan + cv_), x1);
+              }
             } else {
-              atomic_add_g(gptr_w(fd.sum_jones_chan) + c, x0);
-              atomic_add_g(gptr_w(fd.sum_jones_chan) + (v.nchan + c), x1);
+              cd meas[4];
+              if (p.dopsf) {
+                meas[0] = cd{dec, 0}; meas[1] = cd{0, 0}; meas[2] = cd{0, 0}; meas[3] = cd{dec, 0};
+              } else if (READ == 4) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) meas[q] = cd{e[2 * q], e[2 * q + 1]};
+              } else if (READ == 2) {
+                meas[0] = cd{e[0], e[1]}; meas[1] = cd{0, 0}; meas[2] = cd{0, 0}; meas[3] = cd{e[2], e[3]};
+              } else {
+                meas[0] = cd{e[0], e[1]}; meas[1] = cd{0, 0}; meas[2] = cd{0, 0}; meas[3] = meas[0];
+              }
+              cd wt = cd{fw, 0};
+              if (!p.dopsf) wt = ph.at(v, cv_) * fw;
+              if (JMODE == 3) {
+                const JonesPair& jj = jc.get(jp, row, cv_, uvw, p.wl_mean);
+                M2 m;
+                if (p.dopsf) {
+                  m = mul(mul(herm(jj.J0), mul(jj.J0, herm(jj.J1))), jj.J1);
+#pragma unroll
+                  for (int q = 0; q < 4; ++q) m.a[q] = m.a[q] * dec;
+                } else {
+                  m = mul(mul(herm(jj.J0), M2{{meas[0], meas[1], meas[2], meas[3]}}), jj.J1);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) meas[q] = m.a[q];
+                sj += jj.BB * fwd;
+                const double x = double(npg) * (jj.BB * fwd);
+                if (p.chan_lds) atomicAdd(&sch[cv_], x);
+                else atomic_add_g(gptr_w(fd.sum_jones_chan) + cv_, x);
+              }
+              if (JMODE == 2) {
+                const double x = double(npg) * (fwd * BBblk);
+                if (p.chan_lds) atomicAdd(&sch[cv_], x);
+                else atomic_add_g(gptr_w(fd.sum_jones_chan) + cv_, x);
+              }
+              if (JMODE != 0) {
+                const double x = double(npg) * fwd;
+                if (p.chan_lds) atomicAdd(&sch[v.nchan + cv_], x);
+                else atomic_add_g(gptr_w(fd.sum_jones_chan) + (v.nchan + cv_), x);
+              }
+              if (READ == 4) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] = acc[q] + meas[q] * wt;
+              } else if (READ == 2) {
+                acc[0] = acc[0] + meas[0] * wt;
+                acc[3] = acc[3] + meas[3] * wt;
+              } else {
+                acc[0] = acc[0] + meas[0] * wt;
+              }
             }
             swt += fw;
             ssw += fwd;
           }
         }
-        if (any) flush();
-      } else {
-      JonesCache<true> jc;
-      jc.reset();
-      for (int r = 0; r < br.nr; ++r) {
-        const int64_t row = br.rows[r];
-        if (row < 0 || row >= v.nrow) continue;
-        const double* uvw = v.uvw + 3 * row;
-        const DDF_GLOBAL double* guvw = gptr(v.uvw) + 3 * row;
-        ChanPhasor<PREC> ph;
-        ph.row(-2. * kPi * (guvw[0] * fd.l0 + guvw[1] * fd.m0 + guvw[2] * fd.n0) / kC);
-        for (int c = br.ch0; c < br.ch1; ++c) {
-          const size_t vc = size_t(row) * v.nchan + c;
-          if (gptr(v.flags)[vc * v.ncorr]) continue;
-          const double fw = double(gptr(v.weights)[vc]);
-          const double fwd = fw * dec * dec;
-          cd meas[4];
-          if (p.dopsf) {
-            meas[0] = cd{dec, 0}; meas[1] = cd{0, 0}; meas[2] = cd{0, 0}; meas[3] = cd{dec, 0};
-          } else {
-            const DDF_GLOBAL float2* vp = gptr(v.vis) + vc * v.ncorr;
-            if (READ == 4) {
-#pragma unroll
-              for (int q = 0; q < 4; ++q) meas[q] = cd{vp[q].x, vp[q].y};
-            } else if (READ == 2) {
-              meas[0] = cd{vp[0].x, vp[0].y}; meas[1] = cd{0, 0}; meas[2] = cd{0, 0}; meas[3] = cd{vp[1].x, vp[1].y};
-            } else {
-              meas[0] = cd{vp[0].x, vp[0].y}; meas[1] = cd{0, 0}; meas[2] = cd{0, 0}; meas[3] = meas[0];
-            }
-          }
-          cd wt = cd{fw, 0};
-          if (!p.dopsf) wt = ph.at(v, c) * fw;
-          if (JMODE == 1 || JMODE == 3) {
-            const JonesPair& jj = jc.get(jp, row, c, uvw, p.wl_mean);
-            M2 m;
-            if (p.dopsf) {
-              m = mul(mul(herm(jj.J0), mul(jj.J0, herm(jj.J1))), jj.J1);
-#pragma unroll
-              for (int q = 0; q < 4; ++q) m.a[q] = m.a[q] * dec;
-            } else {
-              m = mul(mul(herm(jj.J0), M2{{meas[0], meas[1], meas[2], meas[3]}}), jj.J1);
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) meas[q] = m.a[q];
-            sj += jj.BB * fwd;
-            const double x = double(npg) * (jj.BB * fwd);
-            if (p.chan_lds) atomicAdd(&sch[c], x);
-            else atomic_add_g(gptr_w(fd.sum_jones_chan) + c, x);
-          }
-          if (JMODE == 2) {
-            const double x = double(npg) * (fwd * BBblk);
-            if (p.chan_lds) atomicAdd(&sch[c], x);
-            else atomic_add_g(gptr_w(fd.sum_jones_chan) + c, x);
-          }
-          if (JMODE != 0) {
-            const double x = double(npg) * fwd;
-            if (p.chan_lds) atomicAdd(&sch[v.nchan + c], x);
-            else atomic_add_g(gptr_w(fd.sum_jones_chan) + (v.nchan + c), x);
-          }
-          if (READ == 4) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q] = acc[q] + meas[q] * wt;
-          } else if (READ == 2) {
-            acc[0] = acc[0] + meas[0] * wt;
-            acc[3] = acc[3] + meas[3] * wt;
-          } else {
-            acc[0] = acc[0] + meas[0] * wt;
-          }
-          swt += fw;
-          ssw += fwd;
-        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
       }
-      }
+      if (!doit) continue;
+      if (JMODE == 1 && any) flush();
       if (JMODE == 2) {
         M2 m{{acc[0], acc[1], acc[2], acc[3]}};
         if (p.dopsf) m = mul(mul(jblk.J0, m), herm(jblk.J1));
@@ -979,8 +1018,9 @@ __global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v
       if (JMODE == 2) {
         const int64_t mid = br.rows[br.nr / 2];
         if (mid >= 0 && mid < v.nrow) {
-          const JonesPair jj = jones_at<true>(jp, mid, (br.ch0 + br.ch1) / 2, v.uvw + 3 * mid, p.wl_mean);
-          cv = mul(mul(cvt<RT>(jj.J0), cv), herm(cvt<RT>(jj.J1)));
+          M2T<RT> J0, J1;
+          jones_pair_t<RT>(jp, mid, (br.ch0 + br.ch1) / 2, v.uvw + 3 * mid, p.wl_mean, J0, J1);
+          cv = mul(mul(J0, cv), herm(J1));
         }
       }
       double dec = 1.;
@@ -994,8 +1034,9 @@ __global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v
       const double angle = 2. * kPi * (guvw[0] * fd.l0 + guvw[1] * fd.m0 + guvw[2] * fd.n0) / kC;
       const cx<RT> corr = cvt<RT>(phasor_prec<PREC>(fr, angle) * dec);
       if (JMODE == 1) {
-        const JonesPair jj = jones_at<true>(jp, row, c, uvw, p.wl_mean);
-        cv = mul(mul(cvt<RT>(jj.J0), cv), herm(cvt<RT>(jj.J1)));
+        M2T<RT> J0, J1;
+        jones_pair_t<RT>(jp, row, c, uvw, p.wl_mean, J0, J1);
+        cv = mul(mul(J0, cv), herm(J1));
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[q] = acc[q] + cvt<double>(cv.a[q] * corr);
@@ -1184,9 +1225,15 @@ static void launch_a_j(const BatchParams& p, const VisP& v, const uint64_t* plac
   else launch_a_p<READ, 0>(p, v, place, V, q, lds, nwg, st);
 }
 
+size_t batched_grid_a_stage_bytes(int group, int read) {
+  // (256 / 64) wavefronts x (64 / group) slots x group entries: at most one entry per lane
+  return size_t(256 / 64) * size_t(64 / group) * size_t(group) * size_t(2 * read + 2) * sizeof(float);
+}
+
 int batched_grid_a(const BatchParams& p, const VisP& v, const uint64_t* place, float2* V, uint32_t* queue,
                    unsigned nwg, hipStream_t st) {
-  size_t lds = (size_t(p.nf) * (p.nband * p.npol + 2 * p.nband) + (p.chan_lds ? size_t(p.group) * (2 * v.nchan + 1) : 0)) * 8;
+  size_t lds = (size_t(p.nf) * (p.nband * p.npol + 2 * p.nband) + (p.chan_lds ? size_t(p.group) * (2 * v.nchan + 1) : 0)) * 8 +
+               batched_grid_a_stage_bytes(p.group, p.read);
   if (p.read == 4) launch_a_j<4>(p, v, place, V, queue, lds, nwg, st);
   else if (p.read == 2) launch_a_j<2>(p, v, place, V, queue, lds, nwg, st);
   else launch_a_j<1>(p, v, place, V, queue, lds, nwg, st);
@@ -1333,13 +1380,13 @@ int batched_stamp_records(const BatchParams& p, const uint64_t* place, const uin
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
-int batched_prepare_jones(const BatchParams& p, const float2* src, double2* dst, int dd, hipStream_t st) {
+int batched_prepare_jones(const BatchParams& p, const float2* src, double2* dst, float2* dstf, int dd, hipStream_t st) {
   const int* dims = dd ? p.jp.kdims : p.jp.bdims;
   int64_t total = int64_t(dims[0]) * dims[1] * dims[2] * dims[3];
   int64_t nb = (total + 255) / 256;
   if (nb > 65536) nb = 65536;
   if (nb < 1) nb = 1;
-  hipLaunchKernelGGL(plan_prepare_jones, dim3(unsigned(nb)), dim3(256), 0, st, src, dst, p.jp, dd);
+  hipLaunchKernelGGL(plan_prepare_jones, dim3(unsigned(nb)), dim3(256), 0, st, src, dst, dstf, p.jp, dd);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 

@@ -7,6 +7,8 @@
 // maps it produces (w-plane, grid cell, oversampling offset) are those of the reference.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 #include <stdint.h>
 
 namespace ddf {
@@ -154,6 +156,10 @@ struct JonesP {
   // interpolation applied; null when the raw tables above are read
   const double2* pk;
   const double2* pb;
+  // the same prepared tables rounded to complex64 (read by the FP32 arithmetic of the mixed and
+  // FP32 precisions)
+  const float2* pkf;
+  const float2* pbf;
 };
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -294,6 +300,49 @@ __device__ inline JonesPair jones_at(const JonesP& jp, int64_t row, int chan, co
   double bb = (cabs(J0.a[0]) * cabs(J1.a[0]) + cabs(J0.a[3]) * cabs(J1.a[3])) / 2.;
   r.BB = bb * bb;
   return r;
+}
+
+// Entry of a prepared complex64 table (JonesP::pkf / pbf).
+__device__ __forceinline__ M2T<float> prepared_jones_f(const float2* tab, const int* dims, int it, int ant, int dir,
+                                                       int ich) {
+  it = clampi(it, 0, dims[0] - 1);
+  dir = clampi(dir, 0, dims[1] - 1);
+  ant = clampi(ant, 0, dims[2] - 1);
+  ich = clampi(ich, 0, dims[3] - 1);
+  const DDF_GLOBAL float2* p = gptr(tab) + (((size_t(it) * dims[2] + ant) * size_t(dims[3]) + ich) * size_t(dims[1]) + dir) * 4;
+  M2T<float> out;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out.a[q] = cf{p[q].x, p[q].y};
+  return out;
+}
+
+// The Jones pair of (row, chan) in the arithmetic type T from the prepared tables: FP64 through
+// jones_at; FP32 from the complex64 copies (ScaleJones, which depends on the visibility, goes
+// through the FP64 path).
+template <typename T>
+__device__ __forceinline__ void jones_pair_t(const JonesP& jp, int64_t row, int chan, const double* uvw,
+                                             double wl_mean, M2T<T>& J0, M2T<T>& J1) {
+  if (std::is_same<T, double>::value || jp.scale || !jp.pkf && !jp.pbf) {
+    const JonesPair jj = jones_at<true>(jp, row, chan, uvw, wl_mean);
+    J0 = cvt<T>(jj.J0);
+    J1 = cvt<T>(jj.J1);
+    return;
+  }
+  const int a0 = gptr(jp.a0)[row], a1 = gptr(jp.a1)[row];
+  if (jp.pkf) {
+    const int t = gptr(jp.ktime)[row], c = gptr(jp.kchan)[chan];
+    J0 = cvt<T>(prepared_jones_f(jp.pkf, jp.kdims, t, a0, jp.kdir, c));
+    J1 = cvt<T>(prepared_jones_f(jp.pkf, jp.kdims, t, a1, jp.kdir, c));
+    if (jp.pbf) {
+      const int tb = gptr(jp.btime)[row], cb = gptr(jp.bchan)[chan];
+      J0 = mul(J0, cvt<T>(prepared_jones_f(jp.pbf, jp.bdims, tb, a0, jp.bdir, cb)));
+      J1 = mul(J1, cvt<T>(prepared_jones_f(jp.pbf, jp.bdims, tb, a1, jp.bdir, cb)));
+    }
+  } else {
+    const int tb = gptr(jp.btime)[row], cb = gptr(jp.bchan)[chan];
+    J0 = cvt<T>(prepared_jones_f(jp.pbf, jp.bdims, tb, a0, jp.bdir, cb));
+    J1 = cvt<T>(prepared_jones_f(jp.pbf, jp.bdims, tb, a1, jp.bdir, cb));
+  }
 }
 
 // Per-lane Jones cache with the refresh rule of JonesServer::updateJones (JonesServer.cc:193-298):

@@ -160,6 +160,8 @@ int batched_fill(const BatchParams& p, int64_t nblocks, const uint64_t* place, c
 int batched_vis_counts(const VisP& v, int32_t* cnt, hipStream_t st);
 int batched_vlist(const VisP& v, const int64_t* vis_start, uint64_t* vrc, uint32_t* vblk, hipStream_t st);
 int batched_reset(uint32_t* counters, int n, hipStream_t st);
+// LDS bytes of the visibility staging buffers of gridding pass A (facet group size, corrs read)
+size_t batched_grid_a_stage_bytes(int group, int read);
 int batched_grid_a(const BatchParams& p, const VisP& v, const uint64_t* place, float2* V, uint32_t* queue,
                    unsigned nwg, hipStream_t st);
 // slots: null -> tiles added into the grids with global atomics; else one tile slot per work item
@@ -181,6 +183,6 @@ int batched_tile_wgs(const BatchParams& pg, const BatchParams& pd, unsigned out[
 // stamp records (tile offset | support << 16 | CF offset << 32) of the binned records
 int batched_stamp_records(const BatchParams& p, const uint64_t* place, const uint32_t* recs, int64_t n,
                           uint64_t* bstamp, hipStream_t st);
-int batched_prepare_jones(const BatchParams& p, const float2* src, double2* dst, int dd, hipStream_t st);
+int batched_prepare_jones(const BatchParams& p, const float2* src, double2* dst, float2* dstf, int dd, hipStream_t st);
 
 }  // namespace ddf

@@ -928,6 +928,7 @@ struct ddf_plan {
   uint32_t* queues = nullptr;
   const float2* jsrc[2] = {nullptr, nullptr};  // caller's DD / beam Jones tables
   double2* jprep[2] = {nullptr, nullptr};      // prepared copies (JonesP::pk / pb)
+  float2* jprepf[2] = {nullptr, nullptr};      // complex64 prepared copies (JonesP::pkf / pbf)
   unsigned nwg = 0;
   unsigned nwg_tiles[3] = {0, 0, 0};  // tile passes (grid B, degrid A, adjoint): as many as fit
   int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -963,7 +964,7 @@ namespace {
 
 int prepare_jones(ddf_plan* P, hipStream_t st) {
   for (int k = 0; k < 2; ++k)
-    if (P->jprep[k] && batched_prepare_jones(P->pg, P->jsrc[k], P->jprep[k], k == 0, st)) return DDF_EHIP;
+    if (P->jprep[k] && batched_prepare_jones(P->pg, P->jsrc[k], P->jprep[k], P->jprepf[k], k == 0, st)) return DDF_EHIP;
   return DDF_OK;
 }
 
@@ -1182,12 +1183,19 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
     if (!j.enabled || !src) continue;
     const size_t n = size_t(dims[0]) * dims[1] * dims[2] * dims[3] * 4;
     double2* dst = P->alloc<double2>(n);
-    if (!dst) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
+    float2* dstf = P->alloc<float2>(n);
+    if (!dst || !dstf) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
     P->jsrc[k] = reinterpret_cast<const float2*>(src);
     P->jprep[k] = dst;
+    P->jprepf[k] = dstf;
     for (BatchParams* bp : {&P->pg, &P->pd}) {
-      if (k) bp->jp.pb = dst;
-      else bp->jp.pk = dst;
+      if (k) {
+        bp->jp.pb = dst;
+        bp->jp.pbf = dstf;
+      } else {
+        bp->jp.pk = dst;
+        bp->jp.pkf = dstf;
+      }
     }
   }
   if (prepare_jones(P, st)) return bail(fail(DDF_EHIP, "Jones preparation failed"));
@@ -1214,7 +1222,7 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
     int ng = (nfacets + 31) / 32;
     G = (nfacets + ng - 1) / ng;
   }
-  if (G > 256) G = 256;
+  if (G > 64) G = 64;  // gridding pass A aligns its slots (G lanes each) to wavefronts
   // facet descriptors
   std::vector<FacetDev> hf(static_cast<size_t>(nfacets));
   for (int i = 0; i < nfacets; ++i) {
@@ -1323,8 +1331,13 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   P->pd.nitems_a = uint32_t(int64_t(P->pd.ngroups) * P->pd.nchunks);
   const size_t per_f = size_t(g->nband) * g->npol + 2 * size_t(g->nband);
   if (size_t(nfacets) * per_f * 8 > 40 * 1024) return bail(fail(DDF_EINVAL, "facets x bands too large for one plan"));
+  // pass A LDS: per-facet sums, visibility staging, and (when they fit in 80 KB with the rest,
+  // so that two workgroups share a compute unit) the group's per-channel Jones sums
+  const size_t stage_bytes = batched_grid_a_stage_bytes(G, P->pg.read);
   P->pg.chan_lds = (P->pg.jmode != 0) &&
-                   (size_t(nfacets) * per_f + size_t(G) * (2 * size_t(vgrid->nchan) + 1)) * 8 <= 64 * 1024;
+                   (size_t(nfacets) * per_f + size_t(G) * (2 * size_t(vgrid->nchan) + 1)) * 8 + stage_bytes <= 80 * 1024;
+  if (size_t(nfacets) * per_f * 8 + stage_bytes > 80 * 1024)
+    return bail(fail(DDF_EINVAL, "facets x bands too large for one plan"));
   // LDS tiles
   const int W = tile + 2 * halo;
   const int npg = smg.nstokes < g->npol ? smg.nstokes : g->npol;
