@@ -287,7 +287,10 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
   // lanes (facets) of a wavefront fall on different LDS banks
   const int cs = 2 * v.nchan + 1;
   const int nchan_acc = p.chan_lds ? G * cs : 0;
-  constexpr int kStride = 2 * READ + 2;  // staged entry: READ complex values, weight, flag
+  // staged entry (floats): READ complex values, weight, flag, row, pad, uvw (3 doubles) and for
+  // JMODE 1 the Jones key (DD time, DD chan, beam time, beam chan, antenna 0, antenna 1)
+  constexpr int kW = 2 * READ, kF = kW + 1, kRow = kW + 2, kU = kW + 4, kKey = kU + 6;
+  constexpr int kStride = kKey + (JMODE == 1 ? 6 : 0);
   float* s_stage = reinterpret_cast<float*>(lds + p.nf * per_f + nchan_acc);
   for (int i = threadIdx.x; i < p.nf * per_f + nchan_acc; i += blockDim.x) lds[i] = 0.;
   __syncthreads();
@@ -381,7 +384,7 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
       // in the precision's arithmetic type and the Jones pair applied once per run.
       cx<RT> racc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
       double rfw = 0;  // PSF: sum of the run's weights
-      int kt = -1, kc = -1, bt = -1, bc = -1;
+      int kt = -1, kc = -1, bt = -1, bc = -1, ka0 = -1, ka1 = -1;
       M2T<RT> cJ0, cJ1;
       double cBB = 0;
       bool any = false;
@@ -410,17 +413,15 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
       JonesCache<true> jc;
       jc.reset();
       // row state of the walk
-      int cur_r = -1;
       int64_t row = -1;
-      bool row_ok = false;
-      const double* uvw = v.uvw;
-      int k_t = -1, b_t = -1;
+      double uvwr[3] = {0, 0, 0};
       ChanPhasor<PREC> ph;
       const int nch = br.ch1 - br.ch0;
       const int nvis = br.nr * nch;
       for (int i0 = 0; i0 < nvis; i0 += K) {
         const int n = min(K, nvis - i0);
-        // stage visibilities i0 .. i0 + n - 1 of the block (row-major)
+        // stage visibilities i0 .. i0 + n - 1 of the block (row-major): values, weight, flag,
+        // row, uvw and (JMODE 1) the Jones key, so that the walk below reads LDS only
         for (int i = lf; i < n; i += G) {
           const int g = i0 + i;
           const int rr = g / nch;
@@ -428,55 +429,64 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
           const int64_t srow = br.rows[rr];
           float* e = stg + i * kStride;
           if (srow < 0 || srow >= v.nrow) {
-            e[2 * READ + 1] = 1.f;
+            e[kF] = 1.f;
             continue;
           }
           const size_t vc = size_t(srow) * v.nchan + cc;
           const DDF_GLOBAL float2* vp = gptr(v.vis) + vc * v.ncorr;
+          const DDF_GLOBAL double* q = gptr(v.uvw) + 3 * srow;
 #pragma unroll
-          for (int q = 0; q < READ; ++q) {
-            e[2 * q] = vp[q].x;
-            e[2 * q + 1] = vp[q].y;
+          for (int k = 0; k < READ; ++k) {
+            e[2 * k] = vp[k].x;
+            e[2 * k + 1] = vp[k].y;
           }
-          e[2 * READ] = gptr(v.weights)[vc];
-          e[2 * READ + 1] = gptr(v.flags)[vc * v.ncorr] ? 1.f : 0.f;
+          e[kW] = gptr(v.weights)[vc];
+          e[kF] = gptr(v.flags)[vc * v.ncorr] ? 1.f : 0.f;
+          e[kRow] = __int_as_float(int(srow));
+          double* eu = reinterpret_cast<double*>(e + kU);
+          eu[0] = q[0];
+          eu[1] = q[1];
+          eu[2] = q[2];
+          if (JMODE == 1) {
+            int* ek = reinterpret_cast<int*>(e + kKey);
+            ek[0] = jp.kms ? gptr(jp.ktime)[srow] : -1;
+            ek[1] = jp.kms ? gptr(jp.kchan)[cc] : -1;
+            ek[2] = jp.beam ? gptr(jp.btime)[srow] : -1;
+            ek[3] = jp.beam ? gptr(jp.bchan)[cc] : -1;
+            ek[4] = gptr(jp.a0)[srow];
+            ek[5] = gptr(jp.a1)[srow];
+          }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         if (doit) {
-          int r = i0 / nch;
-          int c = br.ch0 + (i0 - r * nch);
+          int c = br.ch0 + (i0 - (i0 / nch) * nch);
           for (int i = 0; i < n; ++i) {
-            if (r != cur_r) {
-              cur_r = r;
-              row = br.rows[r];
-              row_ok = row >= 0 && row < v.nrow;
-              if (row_ok) {
-                uvw = v.uvw + 3 * row;
-                const DDF_GLOBAL double* guvw = gptr(v.uvw) + 3 * row;
-                ph.row(-2. * kPi * (guvw[0] * fd.l0 + guvw[1] * fd.m0 + guvw[2] * fd.n0) / kC);
-                k_t = jp.kms ? jp.ktime[row] : -1;
-                b_t = jp.beam ? jp.btime[row] : -1;
-              }
-            }
             const float* e = stg + i * kStride;
             const int cv_ = c;
-            if (++c == br.ch1) {
-              c = br.ch0;
-              ++r;
+            if (++c == br.ch1) c = br.ch0;
+            if (e[kF] != 0.f) continue;
+            const int erow = __float_as_int(e[kRow]);
+            if (erow != row) {
+              row = erow;
+              const double* eu = reinterpret_cast<const double*>(e + kU);
+              uvwr[0] = eu[0];
+              uvwr[1] = eu[1];
+              uvwr[2] = eu[2];
+              ph.row(kPhaseScale * (uvwr[0] * fd.l0 + uvwr[1] * fd.m0 + uvwr[2] * fd.n0));
             }
-            if (!row_ok || e[2 * READ + 1] != 0.f) continue;
-            const double fw = double(e[2 * READ]);
+            const double fw = double(e[kW]);
             const double fwd = fw * dec * dec;
             if (JMODE == 1) {
-              const int k_c = jp.kms ? jp.kchan[cv_] : -1, b_c = jp.beam ? jp.bchan[cv_] : -1;
-              if (!any || k_t != kt || k_c != kc || b_t != bt || b_c != bc) {
+              const int* ek = reinterpret_cast<const int*>(e + kKey);
+              const int k_t = ek[0], k_c = ek[1], b_t = ek[2], b_c = ek[3], q0 = ek[4], q1 = ek[5];
+              if (!any || k_t != kt || k_c != kc || b_t != bt || b_c != bc || q0 != ka0 || q1 != ka1) {
                 if (any) flush();
-                const JonesPair jj = jones_at<true>(jp, row, cv_, uvw, p.wl_mean);
+                const JonesPair jj = jones_keyed(jp, q0, q1, k_t, k_c, b_t, b_c);
                 cJ0 = cvt<RT>(jj.J0);
                 cJ1 = cvt<RT>(jj.J1);
                 cBB = jj.BB;
-                kt = k_t; kc = k_c; bt = b_t; bc = b_c;
+                kt = k_t; kc = k_c; bt = b_t; bc = b_c; ka0 = q0; ka1 = q1;
                 any = true;
               }
               if (p.dopsf) {
@@ -520,7 +530,7 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
               cd wt = cd{fw, 0};
               if (!p.dopsf) wt = ph.at(v, cv_) * fw;
               if (JMODE == 3) {
-                const JonesPair& jj = jc.get(jp, row, cv_, uvw, p.wl_mean);
+                const JonesPair& jj = jc.get(jp, row, cv_, uvwr, p.wl_mean);
                 M2 m;
                 if (p.dopsf) {
                   m = mul(mul(herm(jj.J0), mul(jj.J0, herm(jj.J1))), jj.J1);
@@ -1227,7 +1237,7 @@ static void launch_a_j(const BatchParams& p, const VisP& v, const uint64_t* plac
 
 size_t batched_grid_a_stage_bytes(int group, int read) {
   // (256 / 64) wavefronts x (64 / group) slots x group entries: at most one entry per lane
-  return size_t(256 / 64) * size_t(64 / group) * size_t(group) * size_t(2 * read + 2) * sizeof(float);
+  return size_t(256 / 64) * size_t(64 / group) * size_t(group) * size_t(2 * read + 16) * sizeof(float);
 }
 
 int batched_grid_a(const BatchParams& p, const VisP& v, const uint64_t* place, float2* V, uint32_t* queue,

@@ -57,6 +57,8 @@ __device__ __forceinline__ void atomic_add_g(DDF_GLOBAL T* p, T v) {
 
 constexpr double kC = 299792458.0;
 constexpr double kPi = 3.141592653589793238462643383279502884197;
+// exp(i kPhaseScale (u l + v m + w n) nu) is the phasor of a visibility (nu in Hz)
+constexpr double kPhaseScale = -2. * kPi / kC;
 constexpr int kWave = 64;
 
 // ---- complex -------------------------------------------------------------------------------
@@ -298,6 +300,26 @@ __device__ inline JonesPair jones_at(const JonesP& jp, int64_t row, int chan, co
   r.J0 = J0;
   r.J1 = J1;
   double bb = (cabs(J0.a[0]) * cabs(J1.a[0]) + cabs(J0.a[3]) * cabs(J1.a[3])) / 2.;
+  r.BB = bb * bb;
+  return r;
+}
+
+// jones_at<true> for a key read beforehand: antennas a0 / a1, DD Jones time and channel kt / kc,
+// beam Jones time and channel bt / bc (the tables that are absent are not read).  No ScaleJones.
+__device__ __forceinline__ JonesPair jones_keyed(const JonesP& jp, int a0, int a1, int kt, int kc, int bt, int bc) {
+  JonesPair r;
+  if (jp.pk) {
+    r.J0 = prepared_jones(jp.pk, jp.kdims, kt, a0, jp.kdir, kc);
+    r.J1 = prepared_jones(jp.pk, jp.kdims, kt, a1, jp.kdir, kc);
+    if (jp.pb) {
+      r.J0 = mul(r.J0, prepared_jones(jp.pb, jp.bdims, bt, a0, jp.bdir, bc));
+      r.J1 = mul(r.J1, prepared_jones(jp.pb, jp.bdims, bt, a1, jp.bdir, bc));
+    }
+  } else {
+    r.J0 = prepared_jones(jp.pb, jp.bdims, bt, a0, jp.bdir, bc);
+    r.J1 = prepared_jones(jp.pb, jp.bdims, bt, a1, jp.bdir, bc);
+  }
+  const double bb = (cabs(r.J0.a[0]) * cabs(r.J1.a[0]) + cabs(r.J0.a[3]) * cabs(r.J1.a[3])) / 2.;
   r.BB = bb * bb;
   return r;
 }

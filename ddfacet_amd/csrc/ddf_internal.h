@@ -22,6 +22,7 @@ struct VisP {
   int64_t nblocks, bda_len;
   const uint8_t* sparse;
   int equi;  // channels equidistant (LOptimisation[1])
+  double f0, df;  // equidistant channels: first frequency and step (set by the batched plan)
 };
 
 struct BlockRec {
@@ -70,10 +71,11 @@ __device__ __forceinline__ cd phasor_prec(double freq, double angle) {
 }
 
 // Phasors exp(i freq_c angle) along the channels of one row, for a lane that walks them in
-// increasing order.  PREC 0 with equidistant channels: FP64 phase base, FP32 sincos at an anchor
-// (the first channel walked, a channel after a gap, every multiple of 16), and in between the
-// phasor advanced in FP32 by the per-row step exp(i (f1 - f0) angle).  Other precisions, or
-// non-equidistant channels: every phasor formed directly by phasor_prec.
+// increasing order.  PREC 0 with equidistant channels: FP64 phase base (freq_c = f0 + c df, from
+// VisP, so no frequency is loaded), FP32 sincos at an anchor (the first channel walked, a channel
+// after a gap, every multiple of 16), and in between the phasor advanced in FP32 by the per-row
+// step exp(i df angle).  Other precisions, or non-equidistant channels: every phasor formed
+// directly by phasor_prec.
 template <int PREC>
 struct ChanPhasor {
   double angle;
@@ -90,11 +92,11 @@ struct ChanPhasor {
       cr = r;
       ci = i;
     } else {
-      const cd a = phasor(gptr(v.freqs)[c] * angle);
+      const cd a = phasor(fma(double(c), v.df, v.f0) * angle);
       cr = float(a.re);
       ci = float(a.im);
       if (last == -2) {
-        const cd st = phasor((gptr(v.freqs)[1] - gptr(v.freqs)[0]) * angle);
+        const cd st = phasor(v.df * angle);
         sr = float(st.re);
         si = float(st.im);
       }

@@ -1282,6 +1282,16 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   }
   fill_vis(P->vg, vgrid, sg);
   fill_vis(P->vd, vdegrid, sd);
+  // first frequency and step of equidistant channels (the anchors of ChanPhasor)
+  for (VisP* pv : {&P->vg, &P->vd}) {
+    if (!pv->equi) continue;
+    double f2[2] = {0, 0};
+    if (hipMemcpyAsync(f2, pv->freqs, sizeof(f2), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return bail(fail(DDF_EHIP, "reading the channel frequencies failed"));
+    pv->f0 = f2[0];
+    pv->df = f2[1] - f2[0];
+  }
   // placements
   P->place_g = P->alloc<uint64_t>(size_t(vgrid->nblocks) * nfacets);
   P->place_d = P->alloc<uint64_t>(size_t(vdegrid->nblocks) * nfacets);
