@@ -973,99 +973,221 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
 }
 
 // ---- cycle: degrid pass B (lanes over visibilities) ----------------------------------------------
+// A visibility's prediction is sum_f corr_f(vis) Y_f with Y_f = dec_f J0_f S(M_f) J1_f^H, where
+// S(M_f) is the Stokes-to-correlation map of the facet's pass-A value for the block.  Y_f depends
+// on the visibility only through its Jones key, so it is shared by a run of consecutive
+// visibilities of the list with one block and one key (plan_run_ids).  A workgroup takes 256
+// consecutive visibilities; facets go in batches of FB: the workgroup first forms Y_f of every
+// (run of the tile, facet of the batch) into LDS, one pair per lane at a time, and then every
+// lane adds corr_f Y_f of its visibility for the batch's facets.  The sum over facets is FP64; Y_f
+// and the phasor product are in the precision's arithmetic type.
+struct RunInfo {
+  int row, chan, mid;  // representative row and channel; middle row of the block (-1: none)
+  uint32_t blk;
+  double fmean;        // mean frequency of the block's channels (decorrelation)
+};
+
 template <int NCORR, int JMODE, int PREC>
 __global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v, const float2* M,
-                                                     const uint64_t* vrc, const uint32_t* vblk, int64_t nvl) {
+                                                     const uint64_t* vrc, const uint32_t* vblk,
+                                                     const uint32_t* vrun, const uint32_t* run_first, int64_t nvl) {
   using RT = typename std::conditional<PREC == 1, double, float>::type;
-  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < nvl; k += int64_t(gridDim.x) * blockDim.x) {
-    const uint64_t e = vrc[k];
-    const int64_t row = int64_t(uint32_t(e));
-    const int c = int(e >> 32);
-    const int64_t b = vblk[k];
-    if (row < 0 || row >= v.nrow || c < 0 || c >= v.nchan) continue;
-    const double* uvw = v.uvw + 3 * row;
-        const DDF_GLOBAL double* guvw = gptr(v.uvw) + 3 * row;
-    const double fr = gptr(v.freqs)[c];
+  constexpr int FB = PREC == 1 ? 4 : 8;
+  __shared__ cx<RT> sY[FB][256][4];
+  __shared__ uint8_t sNZ[FB][256];
+  __shared__ RunInfo sRun[256];
+  const int t = threadIdx.x;
+  for (int64_t k0 = int64_t(blockIdx.x) * 256; k0 < nvl; k0 += int64_t(gridDim.x) * 256) {
+    const int64_t k = k0 + t;
+    const int64_t klast = min(k0 + 255, nvl - 1);
+    const uint32_t r0 = vrun[k0];
+    const int nr = int(vrun[klast] - r0) + 1;  // runs of the tile (at most 256)
+    // run table of the tile
+    if (t < nr) {
+      RunInfo ri;
+      ri.row = -1;
+      ri.chan = 0;
+      ri.mid = -1;
+      ri.blk = 0;
+      ri.fmean = 0;
+      const uint32_t kr = run_first[r0 + t];
+      if (int64_t(kr) < nvl) {
+        const uint64_t e = vrc[kr];
+        const int64_t rrow = int64_t(uint32_t(e));
+        const int rc_ = int(e >> 32);
+        const uint32_t rb = vblk[kr];
+        if (rrow >= 0 && rrow < v.nrow && rc_ >= 0 && rc_ < v.nchan && int64_t(rb) < v.nblocks) {
+          ri.row = int(rrow);
+          ri.chan = rc_;
+          ri.blk = rb;
+          if (JMODE == 2 || p.dp.enabled) {
+            const BlockRec br = read_block(v, rb);
+            double fm = 0;
+            for (int cc = br.ch0; cc < br.ch1; ++cc) fm += gptr(v.freqs)[cc];
+            if (br.ch1 > br.ch0) fm /= double(br.ch1 - br.ch0);
+            ri.fmean = fm;
+            if (br.nr > 0) {
+              const int64_t mid = br.rows[br.nr / 2];
+              if (mid >= 0 && mid < v.nrow) ri.mid = int(mid);
+            }
+            if (JMODE == 2) ri.chan = (br.ch0 + br.ch1) / 2;
+          }
+        }
+      }
+      sRun[t] = ri;
+    }
+    // this lane's visibility
+    bool live = k < nvl;
+    int64_t row = 0;
+    int c = 0, myrun = 0;
+    if (live) {
+      const uint64_t e = vrc[k];
+      row = int64_t(uint32_t(e));
+      c = int(e >> 32);
+      myrun = int(vrun[k] - r0);
+      live = row >= 0 && row < v.nrow && c >= 0 && c < v.nchan && myrun >= 0 && myrun < nr;
+    }
+    double u0 = 0, u1 = 0, u2 = 0, fr = 0;
+    if (live) {
+      const DDF_GLOBAL double* q = gptr(v.uvw) + 3 * row;
+      u0 = q[0];
+      u1 = q[1];
+      u2 = q[2];
+      fr = gptr(v.freqs)[c];
+    }
     cd acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    BlockRec br;
-    bool have_br = false;
-    double fm = 0;
-    for (int f = 0; f < p.nf; ++f) {
-      const float2* m = M + (size_t(b) * p.nf + f) * p.nstokes;
-      cd st[4];
-      bool any = false;
+    __syncthreads();
+    for (int f0 = 0; f0 < p.nf; f0 += FB) {
+      const int nfb = min(FB, p.nf - f0);
+      // Y of every (run, facet of the batch)
+      for (int j = t; j < nr * nfb; j += 256) {
+        const int fi = j / nr;
+        const int ir = j - fi * nr;
+        const int f = f0 + fi;
+        const RunInfo ri = sRun[ir];
+        int nz = 0;
+        M2T<RT> cv;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (q < p.nstokes) {
-          const float2 x = m[q];
-          st[q] = cd{x.x, x.y};
-          any |= (x.x != 0.f || x.y != 0.f);
+        for (int q = 0; q < 4; ++q) cv.a[q] = cx<RT>{0, 0};
+        if (ri.row >= 0) {
+          const DDF_GLOBAL float2* m = gptr(M) + (size_t(ri.blk) * p.nf + f) * p.nstokes;
+          cd st[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            if (q < p.nstokes) {
+              const float2 x = ld2(m + q);
+              st[q] = cd{x.x, x.y};
+              nz |= (x.x != 0.f || x.y != 0.f);
+            } else {
+              st[q] = cd{0, 0};
+            }
+          }
+          if (nz) {
+            const FacetDev fd = gload(p.facets + f);
+            // Stokes -> correlations and the Jones products in the precision's arithmetic type
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+#pragma unroll
+              for (int s = 0; s < 4; ++s)
+                if (s < p.nstokes) cv.a[q] = cv.a[q] + cx<RT>{RT(p.sm[q][s][0]), RT(p.sm[q][s][1])} * cvt<RT>(st[s]);
+            }
+            JonesP jp = p.jp;
+            jp.kdir = fd.kms_dir;
+            jp.bdir = fd.beam_dir;
+            if (JMODE == 2 && ri.mid >= 0) {
+              M2T<RT> J0, J1;
+              jones_pair_t<RT>(jp, ri.mid, ri.chan, v.uvw + 3 * int64_t(ri.mid), p.wl_mean, J0, J1);
+              cv = mul(mul(J0, cv), herm(J1));
+            }
+            if (JMODE == 1) {
+              M2T<RT> J0, J1;
+              jones_pair_t<RT>(jp, ri.row, ri.chan, v.uvw + 3 * int64_t(ri.row), p.wl_mean, J0, J1);
+              cv = mul(mul(J0, cv), herm(J1));
+            }
+            if (p.dp.enabled && ri.mid >= 0) {
+              DecorrP dp = p.dp;
+              dp.l0 = fd.dec_l0;
+              dp.m0 = fd.dec_m0;
+              const RT dec = RT(decorr_factor(dp, v.uvw, ri.fmean, ri.mid));
+#pragma unroll
+              for (int q = 0; q < 4; ++q) cv.a[q] = cv.a[q] * dec;
+            }
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sY[fi][ir][q] = cv.a[q];
+        sNZ[fi][ir] = uint8_t(nz);
+      }
+      __syncthreads();
+      if (live) {
+        for (int fi = 0; fi < nfb; ++fi) {
+          if (!sNZ[fi][myrun]) continue;
+          const FacetDev fd = gload(p.facets + f0 + fi);
+          const double angle = -kPhaseScale * (u0 * fd.l0 + u1 * fd.m0 + u2 * fd.n0);
+          const cx<RT> corr = cvt<RT>(phasor_prec<PREC>(fr, angle));
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[q] = acc[q] + cvt<double>(sY[fi][myrun][q] * corr);
+        }
+      }
+      __syncthreads();
+    }
+    if (live) {
+      DDF_GLOBAL float2* vp = gptr_w(v.vis) + (size_t(row) * v.nchan + c) * NCORR;
+      if (NCORR == 4) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float2 x = ld2(vp + q);
+          st2(vp + q, make_float2(float(double(x.x) - acc[q].re), float(double(x.y) - acc[q].im)));
+        }
+      } else if (NCORR == 2) {
+        const float2 x0 = ld2(vp), x1 = ld2(vp + 1);
+        st2(vp, make_float2(float(double(x0.x) - acc[0].re), float(double(x0.y) - acc[0].im)));
+        st2(vp + 1, make_float2(float(double(x1.x) - acc[3].re), float(double(x1.y) - acc[3].im)));
+      } else {
+        const float2 x0 = ld2(vp);
+        st2(vp, make_float2(float(double(x0.x) - acc[0].re), float(double(x0.y) - acc[0].im)));
+      }
+    }
+  }
+}
+
+// Run flags of the degrid visibility list: 1 where visibility k starts a new run (another block,
+// or for per-visibility Jones another Jones key: antennas, DD / beam time and channel, and the
+// row itself when ScaleJones makes the Jones depend on uvw).
+__global__ __launch_bounds__(256) void plan_run_flags(const BatchParams p, const VisP v, const uint64_t* vrc,
+                                                      const uint32_t* vblk, int64_t nvl, int32_t* flag) {
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < nvl; k += int64_t(gridDim.x) * blockDim.x) {
+    int fl = 1;
+    if (k > 0 && vblk[k] == vblk[k - 1]) {
+      fl = 0;
+      if (p.jmode == 1) {
+        const uint64_t e1 = vrc[k], e0 = vrc[k - 1];
+        const int64_t r1 = int64_t(uint32_t(e1)), r0 = int64_t(uint32_t(e0));
+        const int c1 = int(e1 >> 32), c0 = int(e0 >> 32);
+        const bool ok = r1 >= 0 && r1 < v.nrow && r0 >= 0 && r0 < v.nrow && c1 >= 0 && c1 < v.nchan && c0 >= 0 &&
+                        c0 < v.nchan;
+        if (!ok) {
+          fl = 1;
         } else {
-          st[q] = cd{0, 0};
+          const JonesP& jp = p.jp;
+          bool diff = jp.a0[r1] != jp.a0[r0] || jp.a1[r1] != jp.a1[r0] || (jp.scale && r1 != r0);
+          if (jp.kms) diff = diff || jp.ktime[r1] != jp.ktime[r0] || jp.kchan[c1] != jp.kchan[c0];
+          if (jp.beam) diff = diff || jp.btime[r1] != jp.btime[r0] || jp.bchan[c1] != jp.bchan[c0];
+          fl = diff ? 1 : 0;
         }
       }
-      if (!any) continue;  // dropped record, or a zero model: contributes exactly 0
-      const FacetDev fd = gload(p.facets + f);
-      // Stokes -> correlations and the Jones products in the precision's arithmetic type (FP32
-      // outside the FP64 precision); the sum over facets stays FP64
-      M2T<RT> cv;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        cv.a[q] = cx<RT>{0, 0};
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-          if (s < p.nstokes) cv.a[q] = cv.a[q] + cx<RT>{RT(p.sm[q][s][0]), RT(p.sm[q][s][1])} * cvt<RT>(st[s]);
-      }
-      JonesP jp = p.jp;
-      jp.kdir = fd.kms_dir;
-      jp.bdir = fd.beam_dir;
-      if ((JMODE == 2 || p.dp.enabled) && !have_br) {
-        br = read_block(v, b);
-        for (int cc = br.ch0; cc < br.ch1; ++cc) fm += gptr(v.freqs)[cc];
-        fm /= double(br.ch1 - br.ch0);
-        have_br = true;
-      }
-      if (JMODE == 2) {
-        const int64_t mid = br.rows[br.nr / 2];
-        if (mid >= 0 && mid < v.nrow) {
-          M2T<RT> J0, J1;
-          jones_pair_t<RT>(jp, mid, (br.ch0 + br.ch1) / 2, v.uvw + 3 * mid, p.wl_mean, J0, J1);
-          cv = mul(mul(J0, cv), herm(J1));
-        }
-      }
-      double dec = 1.;
-      if (p.dp.enabled) {
-        const int64_t mid = br.rows[br.nr / 2];
-        DecorrP dp = p.dp;
-        dp.l0 = fd.dec_l0;
-        dp.m0 = fd.dec_m0;
-        if (mid >= 0 && mid < v.nrow) dec = decorr_factor(dp, v.uvw, fm, mid);
-      }
-      const double angle = 2. * kPi * (guvw[0] * fd.l0 + guvw[1] * fd.m0 + guvw[2] * fd.n0) / kC;
-      const cx<RT> corr = cvt<RT>(phasor_prec<PREC>(fr, angle) * dec);
-      if (JMODE == 1) {
-        M2T<RT> J0, J1;
-        jones_pair_t<RT>(jp, row, c, uvw, p.wl_mean, J0, J1);
-        cv = mul(mul(J0, cv), herm(J1));
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) acc[q] = acc[q] + cvt<double>(cv.a[q] * corr);
     }
-    DDF_GLOBAL float2* vp = gptr_w(v.vis) + (size_t(row) * v.nchan + c) * NCORR;
-    if (NCORR == 4) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float2 x = ld2(vp + q);
-        st2(vp + q, make_float2(float(double(x.x) - acc[q].re), float(double(x.y) - acc[q].im)));
-      }
-    } else if (NCORR == 2) {
-      const float2 x0 = ld2(vp), x1 = ld2(vp + 1);
-      st2(vp, make_float2(float(double(x0.x) - acc[0].re), float(double(x0.y) - acc[0].im)));
-      st2(vp + 1, make_float2(float(double(x1.x) - acc[3].re), float(double(x1.y) - acc[3].im)));
-    } else {
-      const float2 x0 = ld2(vp);
-      st2(vp, make_float2(float(double(x0.x) - acc[0].re), float(double(x0.y) - acc[0].im)));
-    }
+    flag[k] = fl;
+  }
+}
+
+// vrun[k] = run of visibility k (inclusive count of run starts - 1); run_first[run] = its first k.
+__global__ __launch_bounds__(256) void plan_run_ids(const int32_t* flag, const int64_t* excl, int64_t nvl,
+                                                    uint32_t* vrun, uint32_t* run_first) {
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < nvl; k += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r = excl[k] + flag[k] - 1;
+    vrun[k] = uint32_t(r);
+    if (flag[k]) run_first[r] = uint32_t(k);
   }
 }
 
@@ -1115,7 +1237,7 @@ __global__ __launch_bounds__(256) void degrid_adj_b(BatchParams p, const VisP v,
         if (row < 0 || row >= v.nrow) continue;
         const double* uvw = v.uvw + 3 * row;
         const DDF_GLOBAL double* guvw = gptr(v.uvw) + 3 * row;
-        const double angle = 2. * kPi * (guvw[0] * fd.l0 + guvw[1] * fd.m0 + guvw[2] * fd.n0) / kC;
+        const double angle = -kPhaseScale * (guvw[0] * fd.l0 + guvw[1] * fd.m0 + guvw[2] * fd.n0);
         for (int c = br.ch0; c < br.ch1; ++c) {
           const float2* vp = vbar + (size_t(row) * v.nchan + c) * NCORR;
           M2 m;
@@ -1321,28 +1443,43 @@ int batched_degrid_a(const BatchParams& p, const uint64_t* place, float2* M, con
 
 template <int NCORR, int JMODE>
 static void launch_db_p(const BatchParams& p, const VisP& v, const float2* M, const uint64_t* vrc,
-                        const uint32_t* vblk, int64_t nvl, unsigned nb, hipStream_t st) {
-  if (p.prec == 1) hipLaunchKernelGGL((degrid_pass_b<NCORR, JMODE, 1>), dim3(nb), dim3(256), 0, st, p, v, M, vrc, vblk, nvl);
-  else if (p.prec == 2) hipLaunchKernelGGL((degrid_pass_b<NCORR, JMODE, 2>), dim3(nb), dim3(256), 0, st, p, v, M, vrc, vblk, nvl);
-  else hipLaunchKernelGGL((degrid_pass_b<NCORR, JMODE, 0>), dim3(nb), dim3(256), 0, st, p, v, M, vrc, vblk, nvl);
+                        const uint32_t* vblk, const uint32_t* vrun, const uint32_t* run_first, int64_t nvl, unsigned nb,
+                        hipStream_t st) {
+  if (p.prec == 1)
+    hipLaunchKernelGGL((degrid_pass_b<NCORR, JMODE, 1>), dim3(nb), dim3(256), 0, st, p, v, M, vrc, vblk, vrun, run_first, nvl);
+  else if (p.prec == 2)
+    hipLaunchKernelGGL((degrid_pass_b<NCORR, JMODE, 2>), dim3(nb), dim3(256), 0, st, p, v, M, vrc, vblk, vrun, run_first, nvl);
+  else
+    hipLaunchKernelGGL((degrid_pass_b<NCORR, JMODE, 0>), dim3(nb), dim3(256), 0, st, p, v, M, vrc, vblk, vrun, run_first, nvl);
 }
 
 template <int NCORR>
 static void launch_db_j(const BatchParams& p, const VisP& v, const float2* M, const uint64_t* vrc,
-                        const uint32_t* vblk, int64_t nvl, unsigned nb, hipStream_t st) {
-  if (p.jmode == 1) launch_db_p<NCORR, 1>(p, v, M, vrc, vblk, nvl, nb, st);
-  else if (p.jmode == 2) launch_db_p<NCORR, 2>(p, v, M, vrc, vblk, nvl, nb, st);
-  else launch_db_p<NCORR, 0>(p, v, M, vrc, vblk, nvl, nb, st);
+                        const uint32_t* vblk, const uint32_t* vrun, const uint32_t* run_first, int64_t nvl, unsigned nb,
+                        hipStream_t st) {
+  if (p.jmode == 1) launch_db_p<NCORR, 1>(p, v, M, vrc, vblk, vrun, run_first, nvl, nb, st);
+  else if (p.jmode == 2) launch_db_p<NCORR, 2>(p, v, M, vrc, vblk, vrun, run_first, nvl, nb, st);
+  else launch_db_p<NCORR, 0>(p, v, M, vrc, vblk, vrun, run_first, nvl, nb, st);
 }
 
 int batched_degrid_b(const BatchParams& p, const VisP& v, const float2* M, const uint64_t* vrc, const uint32_t* vblk,
-                     int64_t nvl, hipStream_t st) {
+                     const uint32_t* vrun, const uint32_t* run_first, int64_t nvl, hipStream_t st) {
   int64_t nb = (nvl + 255) / 256;
   if (nb > 65536 * 4) nb = 65536 * 4;
   if (nb < 1) nb = 1;
-  if (v.ncorr == 4) launch_db_j<4>(p, v, M, vrc, vblk, nvl, unsigned(nb), st);
-  else if (v.ncorr == 2) launch_db_j<2>(p, v, M, vrc, vblk, nvl, unsigned(nb), st);
-  else launch_db_j<1>(p, v, M, vrc, vblk, nvl, unsigned(nb), st);
+  if (v.ncorr == 4) launch_db_j<4>(p, v, M, vrc, vblk, vrun, run_first, nvl, unsigned(nb), st);
+  else if (v.ncorr == 2) launch_db_j<2>(p, v, M, vrc, vblk, vrun, run_first, nvl, unsigned(nb), st);
+  else launch_db_j<1>(p, v, M, vrc, vblk, vrun, run_first, nvl, unsigned(nb), st);
+  return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
+}
+
+int batched_run_ids(const BatchParams& p, const VisP& v, const uint64_t* vrc, const uint32_t* vblk, int64_t nvl,
+                    int32_t* flag, const int64_t* excl, uint32_t* vrun, uint32_t* run_first, int phase, hipStream_t st) {
+  int64_t nb = (nvl + 255) / 256;
+  if (nb > 65536 * 4) nb = 65536 * 4;
+  if (nb < 1) nb = 1;
+  if (phase == 0) hipLaunchKernelGGL(plan_run_flags, dim3(unsigned(nb)), dim3(256), 0, st, p, v, vrc, vblk, nvl, flag);
+  else hipLaunchKernelGGL(plan_run_ids, dim3(unsigned(nb)), dim3(256), 0, st, flag, excl, nvl, vrun, run_first);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 

@@ -173,8 +173,13 @@ int batched_grid_b(const BatchParams& p, const uint64_t* place, const float2* V,
                    const uint32_t* tile_count, hipStream_t st);
 int batched_degrid_a(const BatchParams& p, const uint64_t* place, float2* M, const BinItem* items,
                      const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st);
+// vrun / run_first: the runs of the degrid visibility list (batched_run_ids)
 int batched_degrid_b(const BatchParams& p, const VisP& v, const float2* M, const uint64_t* vrc, const uint32_t* vblk,
-                     int64_t nvl, hipStream_t st);
+                     const uint32_t* vrun, const uint32_t* run_first, int64_t nvl, hipStream_t st);
+// phase 0: run-start flags of the degrid visibility list; phase 1 (after an exclusive scan of the
+// flags into excl): the run of every visibility and the first visibility of every run
+int batched_run_ids(const BatchParams& p, const VisP& v, const uint64_t* vrc, const uint32_t* vblk, int64_t nvl,
+                    int32_t* flag, const int64_t* excl, uint32_t* vrun, uint32_t* run_first, int phase, hipStream_t st);
 
 int batched_degrid_b_adjoint(const BatchParams& p, const VisP& v, const uint64_t* place, const float2* vbar,
                              float2* Mbar, uint32_t* queue, unsigned nwg, hipStream_t st);

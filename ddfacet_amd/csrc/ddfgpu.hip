@@ -925,6 +925,9 @@ struct ddf_plan {
   uint64_t* vrc = nullptr;
   uint32_t* vblk = nullptr;
   int64_t nvl = 0;
+  uint32_t* vrun = nullptr;       // run of each degrid list entry (block and Jones key shared)
+  uint32_t* run_first = nullptr;  // first list entry of each run
+  int64_t nruns = 0;
   uint32_t* queues = nullptr;
   const float2* jsrc[2] = {nullptr, nullptr};  // caller's DD / beam Jones tables
   double2* jprep[2] = {nullptr, nullptr};      // prepared copies (JonesP::pk / pb)
@@ -1379,6 +1382,30 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
     P->vblk = P->alloc<uint32_t>(size_t(P->nvl));
     if (!P->vrc || !P->vblk) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
     if (batched_vlist(P->vd, vs, P->vrc, P->vblk, st)) return bail(fail(DDF_EHIP, "launch failed"));
+    // runs of consecutive entries with one block and one Jones key: degrid pass B forms each
+    // facet's Jones product once per run
+    if (P->nvl >= (int64_t(1) << 32)) return bail(fail(DDF_EINVAL, "degrid visibility list too long for one plan"));
+    Temp rt(st);
+    int32_t* flag = static_cast<int32_t*>(rt.get(size_t(P->nvl) * sizeof(int32_t)));
+    int64_t* excl = static_cast<int64_t*>(rt.get(size_t(P->nvl) * sizeof(int64_t)));
+    int64_t* ctmp = static_cast<int64_t*>(rt.get(size_t((P->nvl + kScanChunk - 1) / kScanChunk) * sizeof(int64_t)));
+    P->vrun = P->alloc<uint32_t>(size_t(P->nvl));
+    if (!flag || !excl || !ctmp || !P->vrun) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
+    if (batched_run_ids(P->pd, P->vd, P->vrc, P->vblk, P->nvl, flag, nullptr, nullptr, nullptr, 0, st))
+      return bail(fail(DDF_EHIP, "launch failed"));
+    rc = launch_scan(flag, P->nvl, 0, excl, ctmp, st);
+    if (rc) return bail(rc);
+    int64_t last_excl = 0;
+    int32_t last_flag = 0;
+    if (hipMemcpyAsync(&last_excl, excl + P->nvl - 1, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(&last_flag, flag + P->nvl - 1, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return bail(fail(DDF_EHIP, "copy failed"));
+    P->nruns = last_excl + last_flag;
+    P->run_first = P->alloc<uint32_t>(size_t(P->nruns));
+    if (!P->run_first) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
+    if (batched_run_ids(P->pd, P->vd, P->vrc, P->vblk, P->nvl, flag, excl, P->vrun, P->run_first, 1, st))
+      return bail(fail(DDF_EHIP, "launch failed"));
   }
   P->stats[4] = P->nvl;
   P->stats[6] = tile;
@@ -1430,7 +1457,7 @@ extern "C" int ddf_plan_degrid(ddf_plan* P, float* vis, void* stream) {
     return fail(DDF_EHIP, "degrid pass A launch failed");
   P->mark(DDF_PASS_DEGRID_A, 1, st);
   P->mark(DDF_PASS_DEGRID_B, 0, st);
-  if (P->nvl && batched_degrid_b(P->pd, v, P->M, P->vrc, P->vblk, P->nvl, st))
+  if (P->nvl && batched_degrid_b(P->pd, v, P->M, P->vrc, P->vblk, P->vrun, P->run_first, P->nvl, st))
     return fail(DDF_EHIP, "degrid pass B launch failed");
   P->mark(DDF_PASS_DEGRID_B, 1, st);
   return DDF_OK;
