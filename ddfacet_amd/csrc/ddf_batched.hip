@@ -673,17 +673,26 @@ struct Stamp {
   int S, S2, base;
   float invS;
 };
+__device__ __forceinline__ Stamp stamp_none() {
+  Stamp s;
+  s.cfp = nullptr;
+  s.S = s.S2 = s.base = 0;
+  s.invS = 0.f;
+  return s;
+}
 __device__ __forceinline__ Stamp stamp_of(uint64_t srec, const DDF_GLOBAL float2* cf) {
   Stamp s;
   s.base = int(srec & 0xFFFF);
   s.S = int((srec >> 16) & 0x1F);
   s.S2 = s.S * s.S;
   s.cfp = cf + uint32_t(srec >> 32);
-  s.invS = 1.f / float(s.S);
+  // v_rcp_f32 (1 ulp): ample for stamp_cell below
+  s.invS = __builtin_amdgcn_rcpf(float(s.S));
   return s;
 }
-// Tile offset of cell c, or -1 past the stamp.  (c + 0.5) / S in FP32 is at least 0.5 / S from an
-// integer, far above its rounding error for any stamp that fits a tile, so the row is exact.
+// Tile offset of cell c, or -1 past the stamp.  (c + 0.5) / S is at least 0.5 / S from an integer,
+// far above the error of (c + 0.5) * rcp(S) in FP32 for any stamp that fits a tile (c < 1024,
+// S < 32: below 1e-3), so the row is exact.
 __device__ __forceinline__ int stamp_cell(const Stamp& s, int c, int W) {
   const int sy = int((float(c) + 0.5f) * s.invS);
   return c < s.S2 ? sy * W + (c - sy * s.S) : -1;
@@ -695,12 +704,17 @@ struct StampCells {
   float2 w[4];
   int off[4];
 };
+// Groups of 64 cells wholly past the stamp (a uniform test) are neither addressed nor loaded.
 __device__ __forceinline__ void fetch_cells(const Stamp& s, int lane, int W, StampCells& sc) {
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int c = 64 * r + lane;
-    sc.off[r] = stamp_cell(s, c, W);
-    sc.w[r] = ld2(s.cfp + (c < s.S2 ? c : 0));
+    sc.off[r] = -1;
+    sc.w[r] = make_float2(0.f, 0.f);
+    if (64 * r < s.S2) {
+      const int c = 64 * r + lane;
+      sc.off[r] = stamp_cell(s, c, W);
+      sc.w[r] = ld2(s.cfp + (c < s.S2 ? c : 0));
+    }
   }
 }
 
@@ -718,6 +732,7 @@ __device__ __forceinline__ void stamp_add(float2* tile, int WW, int npg, const S
                                           const float* vr, const float* vi) {
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
+    if (64 * r >= s.S2) break;
     if (sc.off[r] < 0) continue;
     const float wx = sc.w[r].x, wy = CONJ ? -sc.w[r].y : sc.w[r].y;
     float2* t = tile + s.base + sc.off[r];
@@ -733,16 +748,18 @@ __device__ __forceinline__ void stamp_add(float2* tile, int WW, int npg, const S
   }
 }
 
-template <bool CONJ>
+// ONE: a single polarisation (npg == 1), with the per-polarisation loops resolved at compile time.
+template <bool CONJ, bool ONE>
 __global__ __launch_bounds__(64) void grid_pass_b(BatchParams p, const uint64_t* bstamp, const float2* V,
                                                   const BinItem* items, const uint32_t* recs, uint32_t* queue,
                                                   float2* slots) {
-  extern __shared__ float lds_f[];
+  // 16-byte aligned: the float2 read-modify-writes below must not straddle 8-byte boundaries
+  extern __shared__ __attribute__((aligned(16))) float lds_f[];
   float2* tile = reinterpret_cast<float2*>(lds_f);  // [npg][W * W]
   __shared__ uint32_t s_item;
   const int W = p.tile + 2 * p.halo;
   const int WW = W * W;
-  const int npg = p.nstokes < p.npol ? p.nstokes : p.npol;
+  const int npg = ONE ? 1 : (p.nstokes < p.npol ? p.nstokes : p.npol);
   const int lane = threadIdx.x;
   while (true) {
     if (lane == 0) s_item = atomicAdd(queue, 1u);
@@ -763,36 +780,47 @@ __global__ __launch_bounds__(64) void grid_pass_b(BatchParams p, const uint64_t*
 #pragma unroll
       for (int ip = 0; ip < 4; ++ip) my_v[ip] = (ok && ip < npg) ? V[my_v0 + ip] : make_float2(0.f, 0.f);
       const int cnt = int(min(64u, bi.end - k0));
-      Stamp cur = stamp_of(lane_u64(my_rec, 0), cf);
-      StampCells cc;
-      fetch_cells(cur, lane, W, cc);
-      for (int j = 0; j < cnt; ++j) {
-        Stamp nxt = cur;
-        StampCells nc = cc;
-        if (j + 1 < cnt) {
-          nxt = stamp_of(lane_u64(my_rec, j + 1), cf);
-          fetch_cells(nxt, lane, W, nc);
+      // Three records in flight: the CF values of record j + 2 are loaded while records j and
+      // j + 1 are added.  The buffers rotate statically (A, B, C), so no register copy has to
+      // wait for a load.
+      Stamp sA = stamp_none(), sB = stamp_none(), sC = stamp_none();
+      StampCells cA, cB, cC;
+      auto load = [&](int j, Stamp& st_, StampCells& c) {
+        if (j < cnt) {
+          st_ = stamp_of(lane_u64(my_rec, j), cf);
+          fetch_cells(st_, lane, W, c);
         }
+      };
+      auto add = [&](int j, const Stamp& st_, const StampCells& c) {
+        if (j >= cnt) return;
         float vr[4], vi[4];
 #pragma unroll
         for (int ip = 0; ip < 4; ++ip) {
           vr[ip] = lane_f32(my_v[ip].x, j);
           vi[ip] = lane_f32(my_v[ip].y, j);
         }
-        stamp_add<CONJ>(tile, WW, npg, cur, cc, vr, vi);
+        stamp_add<CONJ>(tile, WW, npg, st_, c, vr, vi);
         // stamps wider than 16 x 16: the cells past the first 256, without prefetch
-        for (int c0 = 256; c0 < cur.S2; c0 += 256) {
+        for (int c0 = 256; c0 < st_.S2; c0 += 256) {
           StampCells xc;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int c = c0 + 64 * r + lane;
-            xc.off[r] = stamp_cell(cur, c, W);
-            xc.w[r] = ld2(cur.cfp + (c < cur.S2 ? c : 0));
+            const int c1 = c0 + 64 * r + lane;
+            xc.off[r] = stamp_cell(st_, c1, W);
+            xc.w[r] = ld2(st_.cfp + (c1 < st_.S2 ? c1 : 0));
           }
-          stamp_add<CONJ>(tile, WW, npg, cur, xc, vr, vi);
+          stamp_add<CONJ>(tile, WW, npg, st_, xc, vr, vi);
         }
-        cur = nxt;
-        cc = nc;
+      };
+      load(0, sA, cA);
+      load(1, sB, cB);
+      for (int j = 0; j < cnt; j += 3) {
+        load(j + 2, sC, cC);
+        add(j, sA, cA);
+        load(j + 3, sA, cA);
+        add(j + 1, sB, cB);
+        load(j + 4, sB, cB);
+        add(j + 2, sC, cC);
       }
     }
     if (slots) {
@@ -873,6 +901,7 @@ __device__ __forceinline__ void stamp_gather(const float* tile, int WW, int ns, 
                                              ACC* sr, ACC* si) {
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
+    if (64 * r >= s.S2) break;
     if (sc.off[r] < 0) continue;
     const float* t = tile + s.base + sc.off[r];
 #pragma unroll
@@ -938,7 +967,8 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
         StampCells nc = cc;
         if (j + 1 < cnt) {
           nxt = stamp_of(lane_u64(my_rec, j + 1), cf);
-          fetch_cells(nxt, lane, W, nc);
+          // same CF sub-kernel (hence same support): same cells and values
+          if (nxt.cfp != cur.cfp) fetch_cells(nxt, lane, W, nc);
         }
         ACC sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
         stamp_gather(tile, WW, ns, cur, cc, sr, si);
@@ -1388,10 +1418,13 @@ static unsigned persistent_wgs(K kernel, size_t lds, unsigned nwg, int block = 2
 int batched_tile_wgs(const BatchParams& pg, const BatchParams& pd, unsigned out[3]) {
   const int Wg = pg.tile + 2 * pg.halo, Wd = pd.tile + 2 * pd.halo;
   const int npg = pg.nstokes < pg.npol ? pg.nstokes : pg.npol;
-  out[0] = persistent_wgs(grid_pass_b<false>, size_t(npg) * Wg * Wg * sizeof(float2), 0, 64);
+  const int npd = pd.nstokes < pd.npol ? pd.nstokes : pd.npol;
+  out[0] = npg == 1 ? persistent_wgs(grid_pass_b<false, true>, size_t(npg) * Wg * Wg * sizeof(float2), 0, 64)
+                    : persistent_wgs(grid_pass_b<false, false>, size_t(npg) * Wg * Wg * sizeof(float2), 0, 64);
   out[1] = pd.prec == 1 ? persistent_wgs(degrid_pass_a<double>, size_t(pd.nstokes) * Wd * Wd * 2 * sizeof(float), 0)
                         : persistent_wgs(degrid_pass_a<float>, size_t(pd.nstokes) * Wd * Wd * 2 * sizeof(float), 0);
-  out[2] = persistent_wgs(grid_pass_b<true>, size_t(pd.nstokes) * Wd * Wd * sizeof(float2), 0, 64);
+  out[2] = npd == 1 ? persistent_wgs(grid_pass_b<true, true>, size_t(pd.nstokes) * Wd * Wd * sizeof(float2), 0, 64)
+                    : persistent_wgs(grid_pass_b<true, false>, size_t(pd.nstokes) * Wd * Wd * sizeof(float2), 0, 64);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
@@ -1401,8 +1434,12 @@ int batched_grid_b(const BatchParams& p, const uint64_t* place, const float2* V,
   const int W = p.tile + 2 * p.halo;
   const int npg = p.nstokes < p.npol ? p.nstokes : p.npol;
   size_t lds = size_t(npg) * W * W * sizeof(float2);
-  hipLaunchKernelGGL(grid_pass_b<false>, dim3(nwg), dim3(64), lds, st, p, place, V, items, recs, queue,
-                     reinterpret_cast<float2*>(slots));
+  if (npg == 1)
+    hipLaunchKernelGGL((grid_pass_b<false, true>), dim3(nwg), dim3(64), lds, st, p, place, V, items, recs, queue,
+                       reinterpret_cast<float2*>(slots));
+  else
+    hipLaunchKernelGGL((grid_pass_b<false, false>), dim3(nwg), dim3(64), lds, st, p, place, V, items, recs, queue,
+                       reinterpret_cast<float2*>(slots));
   if (hipGetLastError() != hipSuccess) return DDF_EHIP;
   if (slots) {
     const int64_t total = int64_t(p.nf) * p.nband * npg * p.ny * p.nx;
@@ -1422,8 +1459,13 @@ int batched_degrid_a_adjoint(const BatchParams& p, const uint64_t* place, const 
   size_t lds = size_t(p.nstokes) * W * W * sizeof(float2);
   BatchParams q = p;
   q.npol = p.npol;
-  hipLaunchKernelGGL(grid_pass_b<true>, dim3(nwg), dim3(64), lds, st, q, place, Mbar, items, recs, queue,
-                     static_cast<float2*>(nullptr));
+  const int npg = q.nstokes < q.npol ? q.nstokes : q.npol;
+  if (npg == 1)
+    hipLaunchKernelGGL((grid_pass_b<true, true>), dim3(nwg), dim3(64), lds, st, q, place, Mbar, items, recs, queue,
+                       static_cast<float2*>(nullptr));
+  else
+    hipLaunchKernelGGL((grid_pass_b<true, false>), dim3(nwg), dim3(64), lds, st, q, place, Mbar, items, recs, queue,
+                       static_cast<float2*>(nullptr));
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
@@ -1517,6 +1559,68 @@ int batched_degrid_b_adjoint(const BatchParams& p, const VisP& v, const uint64_t
 }  // namespace ddf
 
 namespace ddf {
+
+// Order of the records inside each bin item by CF sub-kernel (the CF offset of the stamp record),
+// ties by position: the tile passes then load a sub-kernel's CF values once for a run of records
+// that share it.  One workgroup per item; items longer than kSortMax keep their order.
+constexpr int kSortMax = 4096;
+__global__ __launch_bounds__(256) void plan_sort_items(const BinItem* items, uint32_t nitems, const uint64_t* bstamp,
+                                                       uint32_t* perm) {
+  __shared__ uint64_t key[kSortMax];
+  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const BinItem bi = items[it];
+    const int n = int(bi.end - bi.begin);
+    if (n > kSortMax) {
+      for (int i = threadIdx.x; i < n; i += blockDim.x) perm[bi.begin + i] = bi.begin + uint32_t(i);
+      continue;
+    }
+    int P = 1;
+    while (P < n) P <<= 1;
+    for (int i = threadIdx.x; i < P; i += blockDim.x)
+      key[i] = i < n ? ((bstamp[bi.begin + i] >> 32) << 32) | uint64_t(i) : ~uint64_t(0);
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < P; i += blockDim.x) {
+          const int l = i ^ j;
+          if (l > i) {
+            const uint64_t a = key[i], b = key[l];
+            if ((a > b) == ((i & k) == 0)) {
+              key[i] = b;
+              key[l] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) perm[bi.begin + i] = bi.begin + uint32_t(key[i] & 0xFFFFFFFFu);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void plan_permute(const uint32_t* perm, int64_t n, const uint64_t* bs_in,
+                                                    const uint32_t* rec_in, uint64_t* bs_out, uint32_t* rec_out) {
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t q = perm[k];
+    bs_out[k] = bs_in[q];
+    rec_out[k] = rec_in[q];
+  }
+}
+
+int batched_sort_items(const BinItem* items, uint32_t nitems, int64_t n, uint64_t* bstamp, uint32_t* recs,
+                       uint32_t* perm, uint64_t* bs_tmp, uint32_t* rec_tmp, hipStream_t st) {
+  if (nitems == 0 || n == 0) return DDF_OK;
+  const unsigned nb = nitems < 65536u ? nitems : 65536u;
+  hipLaunchKernelGGL(plan_sort_items, dim3(nb), dim3(256), 0, st, items, nitems, bstamp, perm);
+  if (hipMemcpyAsync(bs_tmp, bstamp, size_t(n) * 8, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(rec_tmp, recs, size_t(n) * 4, hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return DDF_EHIP;
+  int64_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(plan_permute, dim3(unsigned(g)), dim3(256), 0, st, perm, n, bs_tmp, rec_tmp, bstamp, recs);
+  return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
+}
 
 int batched_stamp_records(const BatchParams& p, const uint64_t* place, const uint32_t* recs, int64_t n,
                           uint64_t* bstamp, hipStream_t st) {
