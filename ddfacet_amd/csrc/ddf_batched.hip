@@ -920,14 +920,15 @@ __device__ __forceinline__ float wave_sum_acc(float x) { return wave_total_f32(x
 // ACC: the accumulator of the stamp sums (double as the reference; float in the mixed and FP32
 // precisions, where the FP32 products of one stamp are summed in FP32 per lane and across the
 // wavefront)
-template <typename ACC>
+// ONE: a single Stokes plane (nstokes == 1), resolved at compile time.
+template <typename ACC, bool ONE>
 __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64_t* bstamp, float2* M,
                                                      const BinItem* items, const uint32_t* recs, uint32_t* queue) {
   extern __shared__ float tile[];  // [nstokes][2][W * W]
   __shared__ uint32_t s_item;
   const int W = p.tile + 2 * p.halo;
   const int WW = W * W;
-  const int ns = p.nstokes;
+  const int ns = ONE ? 1 : p.nstokes;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nwave = blockDim.x >> 6;
@@ -959,28 +960,28 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
       const uint32_t my_t = ok ? recs[k] : 0;
       float2 res[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
       const int cnt = int(min(64u, bi.end - k0));
-      Stamp cur = stamp_of(lane_u64(my_rec, 0), cf);
-      StampCells cc;
-      fetch_cells(cur, lane, W, cc);
-      for (int j = 0; j < cnt; ++j) {
-        Stamp nxt = cur;
-        StampCells nc = cc;
-        if (j + 1 < cnt) {
-          nxt = stamp_of(lane_u64(my_rec, j + 1), cf);
-          // same CF sub-kernel (hence same support): same cells and values
-          if (nxt.cfp != cur.cfp) fetch_cells(nxt, lane, W, nc);
+      // three records in flight, buffers rotated statically (as in grid_pass_b)
+      Stamp sA = stamp_none(), sB = stamp_none(), sC = stamp_none();
+      StampCells cA, cB, cC;
+      auto load = [&](int j, Stamp& st_, StampCells& c) {
+        if (j < cnt) {
+          st_ = stamp_of(lane_u64(my_rec, j), cf);
+          fetch_cells(st_, lane, W, c);
         }
+      };
+      auto gather = [&](int j, const Stamp& st_, const StampCells& c) {
+        if (j >= cnt) return;
         ACC sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
-        stamp_gather(tile, WW, ns, cur, cc, sr, si);
-        for (int c0 = 256; c0 < cur.S2; c0 += 256) {
+        stamp_gather(tile, WW, ns, st_, c, sr, si);
+        for (int c0 = 256; c0 < st_.S2; c0 += 256) {
           StampCells xc;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int c = c0 + 64 * r + lane;
-            xc.off[r] = stamp_cell(cur, c, W);
-            xc.w[r] = ld2(cur.cfp + (c < cur.S2 ? c : 0));
+            const int c1 = c0 + 64 * r + lane;
+            xc.off[r] = stamp_cell(st_, c1, W);
+            xc.w[r] = ld2(st_.cfp + (c1 < st_.S2 ? c1 : 0));
           }
-          stamp_gather(tile, WW, ns, cur, xc, sr, si);
+          stamp_gather(tile, WW, ns, st_, xc, sr, si);
         }
 #pragma unroll
         for (int ip = 0; ip < 4; ++ip) {
@@ -989,8 +990,16 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
             if (lane == j) res[ip] = make_float2(float(a), float(b));
           }
         }
-        cur = nxt;
-        cc = nc;
+      };
+      load(0, sA, cA);
+      load(1, sB, cB);
+      for (int j = 0; j < cnt; j += 3) {
+        load(j + 2, sC, cC);
+        gather(j, sA, cA);
+        load(j + 3, sA, cA);
+        gather(j + 1, sB, cB);
+        load(j + 4, sB, cB);
+        gather(j + 2, sC, cC);
       }
       if (ok) {
 #pragma unroll
@@ -1421,8 +1430,13 @@ int batched_tile_wgs(const BatchParams& pg, const BatchParams& pd, unsigned out[
   const int npd = pd.nstokes < pd.npol ? pd.nstokes : pd.npol;
   out[0] = npg == 1 ? persistent_wgs(grid_pass_b<false, true>, size_t(npg) * Wg * Wg * sizeof(float2), 0, 64)
                     : persistent_wgs(grid_pass_b<false, false>, size_t(npg) * Wg * Wg * sizeof(float2), 0, 64);
-  out[1] = pd.prec == 1 ? persistent_wgs(degrid_pass_a<double>, size_t(pd.nstokes) * Wd * Wd * 2 * sizeof(float), 0)
-                        : persistent_wgs(degrid_pass_a<float>, size_t(pd.nstokes) * Wd * Wd * 2 * sizeof(float), 0);
+  const size_t lds_d = size_t(pd.nstokes) * Wd * Wd * 2 * sizeof(float);
+  if (pd.nstokes == 1)
+    out[1] = pd.prec == 1 ? persistent_wgs(degrid_pass_a<double, true>, lds_d, 0)
+                          : persistent_wgs(degrid_pass_a<float, true>, lds_d, 0);
+  else
+    out[1] = pd.prec == 1 ? persistent_wgs(degrid_pass_a<double, false>, lds_d, 0)
+                          : persistent_wgs(degrid_pass_a<float, false>, lds_d, 0);
   out[2] = npd == 1 ? persistent_wgs(grid_pass_b<true, true>, size_t(pd.nstokes) * Wd * Wd * sizeof(float2), 0, 64)
                     : persistent_wgs(grid_pass_b<true, false>, size_t(pd.nstokes) * Wd * Wd * sizeof(float2), 0, 64);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
@@ -1469,16 +1483,23 @@ int batched_degrid_a_adjoint(const BatchParams& p, const uint64_t* place, const 
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
+template <typename ACC, bool ONE>
+static void launch_degrid_a(const BatchParams& p, const uint64_t* place, float2* M, const BinItem* items,
+                            const uint32_t* recs, uint32_t* queue, unsigned nwg, size_t lds, hipStream_t st) {
+  nwg = persistent_wgs(degrid_pass_a<ACC, ONE>, lds, nwg);
+  hipLaunchKernelGGL((degrid_pass_a<ACC, ONE>), dim3(nwg), dim3(256), lds, st, p, place, M, items, recs, queue);
+}
+
 int batched_degrid_a(const BatchParams& p, const uint64_t* place, float2* M, const BinItem* items,
                      const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st) {
   const int W = p.tile + 2 * p.halo;
   size_t lds = size_t(p.nstokes) * W * W * 2 * sizeof(float);
   if (p.prec == 1) {
-    nwg = persistent_wgs(degrid_pass_a<double>, lds, nwg);
-    hipLaunchKernelGGL(degrid_pass_a<double>, dim3(nwg), dim3(256), lds, st, p, place, M, items, recs, queue);
+    if (p.nstokes == 1) launch_degrid_a<double, true>(p, place, M, items, recs, queue, nwg, lds, st);
+    else launch_degrid_a<double, false>(p, place, M, items, recs, queue, nwg, lds, st);
   } else {
-    nwg = persistent_wgs(degrid_pass_a<float>, lds, nwg);
-    hipLaunchKernelGGL(degrid_pass_a<float>, dim3(nwg), dim3(256), lds, st, p, place, M, items, recs, queue);
+    if (p.nstokes == 1) launch_degrid_a<float, true>(p, place, M, items, recs, queue, nwg, lds, st);
+    else launch_degrid_a<float, false>(p, place, M, items, recs, queue, nwg, lds, st);
   }
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
