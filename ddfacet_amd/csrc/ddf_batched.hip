@@ -383,7 +383,9 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
       // the sum of the per-visibility products (gridder.h:255-260): the run sum is accumulated
       // in the precision's arithmetic type and the Jones pair applied once per run.
       cx<RT> racc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-      double rfw = 0;  // PSF: sum of the run's weights
+      double rfw = 0;  // sum of the run's weights
+      double sjr = 0;  // JMODE 1: sum of BB w over the block's runs
+      const double dec2 = dec * dec, ndec2 = double(npg) * dec2;
       int kt = -1, kc = -1, bt = -1, bc = -1, ka0 = -1, ka1 = -1;
       M2T<RT> cJ0, cJ1;
       double cBB = 0;
@@ -408,6 +410,9 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) racc[q] = cx<RT>{0, 0};
+        // weight sums of the run: sum w, and sum BB w (dec^2 applied at the block end)
+        swt += rfw;
+        sjr += cBB * rfw;
         rfw = 0;
       };
       JonesCache<true> jc;
@@ -476,7 +481,7 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
               ph.row(kPhaseScale * (uvwr[0] * fd.l0 + uvwr[1] * fd.m0 + uvwr[2] * fd.n0));
             }
             const double fw = double(e[kW]);
-            const double fwd = fw * dec * dec;
+            const double fwd = fw * dec2;
             if (JMODE == 1) {
               const int* ek = reinterpret_cast<const int*>(e + kKey);
               const int k_t = ek[0], k_c = ek[1], b_t = ek[2], b_c = ek[3], q0 = ek[4], q1 = ek[5];
@@ -489,10 +494,9 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
                 kt = k_t; kc = k_c; bt = b_t; bc = b_c; ka0 = q0; ka1 = q1;
                 any = true;
               }
-              if (p.dopsf) {
-                rfw += fw;
-              } else {
-                const cx<RT> wt = cvt<RT>(ph.at(v, cv_) * fw);
+              rfw += fw;
+              if (!p.dopsf) {
+                const cx<RT> wt = ph.template weighted<RT>(v, cv_, e[kW]);
                 if (READ == 4) {
 #pragma unroll
                   for (int q = 0; q < 4; ++q) racc[q] = racc[q] + cx<RT>{RT(e[2 * q]), RT(e[2 * q + 1])} * wt;
@@ -505,9 +509,7 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
                   racc[3] = racc[3] + x;
                 }
               }
-              const double bbw = cBB * fwd;
-              sj += bbw;
-              const double x0 = double(npg) * bbw, x1 = double(npg) * fwd;
+              const double x1 = ndec2 * fw, x0 = cBB * x1;
               if (p.chan_lds) {
                 atomicAdd(&sch[cv_], x0);
                 atomicAdd(&sch[v.nchan + cv_], x1);
@@ -566,15 +568,21 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
                 acc[0] = acc[0] + meas[0] * wt;
               }
             }
-            swt += fw;
-            ssw += fwd;
+            if (JMODE != 1) {
+              swt += fw;
+              ssw += fwd;
+            }
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
       }
       if (!doit) continue;
-      if (JMODE == 1 && any) flush();
+      if (JMODE == 1) {
+        if (any) flush();
+        ssw = dec2 * swt;
+        sj = dec2 * sjr;
+      }
       if (JMODE == 2) {
         M2 m{{acc[0], acc[1], acc[2], acc[3]}};
         if (p.dopsf) m = mul(mul(jblk.J0, m), herm(jblk.J1));

@@ -104,7 +104,23 @@ struct ChanPhasor {
     last = c;
     return cd{cr, ci};
   }
+  // phasor x weight in the arithmetic type T (float: the FP32 phasor times the FP32 weight, the
+  // same correctly rounded product as forming it in FP64 and rounding)
+  template <typename T>
+  __device__ __forceinline__ cx<T> weighted(const VisP& v, int c, float w);
 };
+template <int PREC>
+template <typename T>
+__device__ __forceinline__ cx<T> ChanPhasor<PREC>::weighted(const VisP& v, int c, float w) {
+  if (std::is_same<T, double>::value || PREC == 1 || (PREC == 0 && !v.equi)) return cvt<T>(at(v, c) * double(w));
+  if (PREC == 2) {
+    float s, co;
+    sincosf(float(gptr(v.freqs)[c]) * float(angle), &s, &co);
+    return cx<T>{T(co * w), T(s * w)};
+  }
+  (void)at(v, c);
+  return cx<T>{T(cr * w), T(ci * w)};
+}
 
 // One facet as the batched kernels see it.
 struct FacetDev {
