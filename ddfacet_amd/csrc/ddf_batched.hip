@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
   // weight, flag) in LDS, G at a time, one per lane, and then every lane walks the staged
   // visibilities in the reference order.  A wavefront's LDS operations complete in order,
   // so the staging needs no workgroup barrier.
-  extern __shared__ double lds[];
+  extern __shared__ __attribute__((aligned(16))) double lds[];
   const int G = p.group;
   const int spw = 64 / G;
   const int nbp = p.nband * p.npol;
@@ -289,9 +289,11 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
   const int nchan_acc = p.chan_lds ? G * cs : 0;
   // staged entry (floats): READ complex values, weight, flag, row, pad, uvw (3 doubles) and for
   // JMODE 1 the Jones key (DD time, DD chan, beam time, beam chan, antenna 0, antenna 1)
-  constexpr int kW = 2 * READ, kF = kW + 1, kRow = kW + 2, kU = kW + 4, kKey = kU + 6;
-  constexpr int kStride = kKey + (JMODE == 1 ? 6 : 0);
-  float* s_stage = reinterpret_cast<float*>(lds + p.nf * per_f + nchan_acc);
+  // (weight, flag, row, pad) first, so that the walk reads an entry with a few 16-byte LDS reads
+  constexpr int kW = 0, kF = 1, kRow = 2, kV = 4;
+  constexpr int kKey = grid_a_key_offset(READ), kU = kKey + 6, kStride = grid_a_stride(READ);
+  // 16-byte aligned staging buffers
+  float* s_stage = reinterpret_cast<float*>(lds + ((p.nf * per_f + nchan_acc + 1) & ~1));
   for (int i = threadIdx.x; i < p.nf * per_f + nchan_acc; i += blockDim.x) lds[i] = 0.;
   __syncthreads();
   const int wl = threadIdx.x & 63;
@@ -442,8 +444,8 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
           const DDF_GLOBAL double* q = gptr(v.uvw) + 3 * srow;
 #pragma unroll
           for (int k = 0; k < READ; ++k) {
-            e[2 * k] = vp[k].x;
-            e[2 * k + 1] = vp[k].y;
+            e[kV + 2 * k] = vp[k].x;
+            e[kV + 2 * k + 1] = vp[k].y;
           }
           e[kW] = gptr(v.weights)[vc];
           e[kF] = gptr(v.flags)[vc * v.ncorr] ? 1.f : 0.f;
@@ -468,10 +470,31 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
           int c = br.ch0 + (i0 - (i0 / nch) * nch);
           for (int i = 0; i < n; ++i) {
             const float* e = stg + i * kStride;
+            // the whole entry in one batch of LDS reads (one wait)
+            const float4 hd = *reinterpret_cast<const float4*>(e);
+            float ev[8];
+            if (READ == 4) {
+              const float4 a = *reinterpret_cast<const float4*>(e + kV);
+              const float4 b4 = *reinterpret_cast<const float4*>(e + kV + 4);
+              ev[0] = a.x; ev[1] = a.y; ev[2] = a.z; ev[3] = a.w;
+              ev[4] = b4.x; ev[5] = b4.y; ev[6] = b4.z; ev[7] = b4.w;
+            } else if (READ == 2) {
+              const float4 a = *reinterpret_cast<const float4*>(e + kV);
+              ev[0] = a.x; ev[1] = a.y; ev[2] = a.z; ev[3] = a.w;
+            } else {
+              const float2 a = *reinterpret_cast<const float2*>(e + kV);
+              ev[0] = a.x; ev[1] = a.y;
+            }
+            int4 ka = make_int4(0, 0, 0, 0);
+            int2 kb = make_int2(0, 0);
+            if (JMODE == 1) {
+              ka = *reinterpret_cast<const int4*>(e + kKey);
+              kb = *reinterpret_cast<const int2*>(e + kKey + 4);
+            }
             const int cv_ = c;
             if (++c == br.ch1) c = br.ch0;
-            if (e[kF] != 0.f) continue;
-            const int erow = __float_as_int(e[kRow]);
+            if (hd.y != 0.f) continue;
+            const int erow = __float_as_int(hd.z);
             if (erow != row) {
               row = erow;
               const double* eu = reinterpret_cast<const double*>(e + kU);
@@ -480,11 +503,10 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
               uvwr[2] = eu[2];
               ph.row(kPhaseScale * (uvwr[0] * fd.l0 + uvwr[1] * fd.m0 + uvwr[2] * fd.n0));
             }
-            const double fw = double(e[kW]);
+            const double fw = double(hd.x);
             const double fwd = fw * dec2;
             if (JMODE == 1) {
-              const int* ek = reinterpret_cast<const int*>(e + kKey);
-              const int k_t = ek[0], k_c = ek[1], b_t = ek[2], b_c = ek[3], q0 = ek[4], q1 = ek[5];
+              const int k_t = ka.x, k_c = ka.y, b_t = ka.z, b_c = ka.w, q0 = kb.x, q1 = kb.y;
               if (!any || k_t != kt || k_c != kc || b_t != bt || b_c != bc || q0 != ka0 || q1 != ka1) {
                 if (any) flush();
                 const JonesPair jj = jones_keyed(jp, q0, q1, k_t, k_c, b_t, b_c);
@@ -496,15 +518,15 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
               }
               rfw += fw;
               if (!p.dopsf) {
-                const cx<RT> wt = ph.template weighted<RT>(v, cv_, e[kW]);
+                const cx<RT> wt = ph.template weighted<RT>(v, cv_, hd.x);
                 if (READ == 4) {
 #pragma unroll
-                  for (int q = 0; q < 4; ++q) racc[q] = racc[q] + cx<RT>{RT(e[2 * q]), RT(e[2 * q + 1])} * wt;
+                  for (int q = 0; q < 4; ++q) racc[q] = racc[q] + cx<RT>{RT(ev[2 * q]), RT(ev[2 * q + 1])} * wt;
                 } else if (READ == 2) {
-                  racc[0] = racc[0] + cx<RT>{RT(e[0]), RT(e[1])} * wt;
-                  racc[3] = racc[3] + cx<RT>{RT(e[2]), RT(e[3])} * wt;
+                  racc[0] = racc[0] + cx<RT>{RT(ev[0]), RT(ev[1])} * wt;
+                  racc[3] = racc[3] + cx<RT>{RT(ev[2]), RT(ev[3])} * wt;
                 } else {
-                  const cx<RT> x = cx<RT>{RT(e[0]), RT(e[1])} * wt;
+                  const cx<RT> x = cx<RT>{RT(ev[0]), RT(ev[1])} * wt;
                   racc[0] = racc[0] + x;
                   racc[3] = racc[3] + x;
                 }
@@ -523,11 +545,11 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
                 meas[0] = cd{dec, 0}; meas[1] = cd{0, 0}; meas[2] = cd{0, 0}; meas[3] = cd{dec, 0};
               } else if (READ == 4) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) meas[q] = cd{e[2 * q], e[2 * q + 1]};
+                for (int q = 0; q < 4; ++q) meas[q] = cd{ev[2 * q], ev[2 * q + 1]};
               } else if (READ == 2) {
-                meas[0] = cd{e[0], e[1]}; meas[1] = cd{0, 0}; meas[2] = cd{0, 0}; meas[3] = cd{e[2], e[3]};
+                meas[0] = cd{ev[0], ev[1]}; meas[1] = cd{0, 0}; meas[2] = cd{0, 0}; meas[3] = cd{ev[2], ev[3]};
               } else {
-                meas[0] = cd{e[0], e[1]}; meas[1] = cd{0, 0}; meas[2] = cd{0, 0}; meas[3] = meas[0];
+                meas[0] = cd{ev[0], ev[1]}; meas[1] = cd{0, 0}; meas[2] = cd{0, 0}; meas[3] = meas[0];
               }
               cd wt = cd{fw, 0};
               if (!p.dopsf) wt = ph.at(v, cv_) * fw;
@@ -1406,7 +1428,8 @@ static void launch_a_j(const BatchParams& p, const VisP& v, const uint64_t* plac
 
 size_t batched_grid_a_stage_bytes(int group, int read) {
   // (256 / 64) wavefronts x (64 / group) slots x group entries: at most one entry per lane
-  return size_t(256 / 64) * size_t(64 / group) * size_t(group) * size_t(2 * read + 16) * sizeof(float);
+  // plus 16 bytes for the alignment of the staging area
+  return size_t(256 / 64) * size_t(64 / group) * size_t(group) * size_t(grid_a_stride(read)) * sizeof(float) + 16;
 }
 
 int batched_grid_a(const BatchParams& p, const VisP& v, const uint64_t* place, float2* V, uint32_t* queue,

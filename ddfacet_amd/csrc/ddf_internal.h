@@ -170,6 +170,12 @@ struct BatchParams {
   uint32_t nitems_b;
 };
 
+// Staged visibility entry of gridding pass A (floats): weight, flag, row, pad; READ complex
+// values from offset 4; the Jones key (6 ints) from grid_a_key_offset; uvw (3 doubles) after it;
+// the stride a multiple of 4 floats so that every entry is 16-byte aligned.
+constexpr int grid_a_key_offset(int read) { return (4 + 2 * read + 3) & ~3; }
+constexpr int grid_a_stride(int read) { return (grid_a_key_offset(read) + 12 + 3) & ~3; }
+
 // host launchers (ddf_batched.hip)
 int batched_place(const BatchParams& p, const VisP& v, bool degrid, uint64_t* place, hipStream_t st);
 int batched_count(const BatchParams& p, int64_t nblocks, const uint64_t* place, uint32_t* counts, hipStream_t st);
