@@ -613,7 +613,8 @@ __global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, 
         for (int q = 0; q < 4; ++q) acc[q] = m.a[q];
         sj = ssw * BBblk;
       }
-      float2* out = V + (size_t(b) * p.nf + f) * p.nstokes;
+      // facet-major [f][block]: the tile pass reads one facet's records in block order
+      float2* out = V + (size_t(f) * v.nblocks + b) * p.nstokes;
       for (int k = 0; k < p.nstokes; ++k) {
         cd s = cd{0, 0};
 #pragma unroll
@@ -805,7 +806,9 @@ __global__ __launch_bounds__(64) void grid_pass_b(BatchParams p, const uint64_t*
       const uint32_t k = k0 + lane;
       const bool ok = k < bi.end;
       const uint64_t my_rec = ok ? bstamp[k] : 0;
-      const size_t my_v0 = ok ? size_t(recs[k]) * p.nstokes : 0;
+      // record t = block * nf + facet; its value at [facet][block]
+      const uint32_t my_t = ok ? recs[k] : 0;
+      const size_t my_v0 = (size_t(bi.facet) * p.nblk + my_t / uint32_t(p.nf)) * p.nstokes;
       float2 my_v[4];
 #pragma unroll
       for (int ip = 0; ip < 4; ++ip) my_v[ip] = (ok && ip < npg) ? V[my_v0 + ip] : make_float2(0.f, 0.f);
@@ -1034,7 +1037,7 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
       if (ok) {
 #pragma unroll
         for (int ip = 0; ip < 4; ++ip)
-          if (ip < ns) M[size_t(my_t) * ns + ip] = res[ip];
+          if (ip < ns) M[(size_t(bi.facet) * p.nblk + my_t / uint32_t(p.nf)) * ns + ip] = res[ip];
       }
     }
     __syncthreads();
@@ -1139,7 +1142,7 @@ __global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v
 #pragma unroll
         for (int q = 0; q < 4; ++q) cv.a[q] = cx<RT>{0, 0};
         if (ri.row >= 0) {
-          const DDF_GLOBAL float2* m = gptr(M) + (size_t(ri.blk) * p.nf + f) * p.nstokes;
+          const DDF_GLOBAL float2* m = gptr(M) + (size_t(f) * v.nblocks + ri.blk) * p.nstokes;
           cd st[4];
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
@@ -1334,7 +1337,7 @@ __global__ __launch_bounds__(256) void degrid_adj_b(BatchParams p, const VisP v,
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = m.a[q];
       }
-      float2* out = Mbar + (size_t(b) * p.nf + f) * p.nstokes;
+      float2* out = Mbar + (size_t(f) * v.nblocks + b) * p.nstokes;
       for (int s = 0; s < p.nstokes; ++s) {
         cd t = cd{0, 0};
 #pragma unroll
@@ -1611,68 +1614,6 @@ int batched_degrid_b_adjoint(const BatchParams& p, const VisP& v, const uint64_t
 }  // namespace ddf
 
 namespace ddf {
-
-// Order of the records inside each bin item by CF sub-kernel (the CF offset of the stamp record),
-// ties by position: the tile passes then load a sub-kernel's CF values once for a run of records
-// that share it.  One workgroup per item; items longer than kSortMax keep their order.
-constexpr int kSortMax = 4096;
-__global__ __launch_bounds__(256) void plan_sort_items(const BinItem* items, uint32_t nitems, const uint64_t* bstamp,
-                                                       uint32_t* perm) {
-  __shared__ uint64_t key[kSortMax];
-  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const BinItem bi = items[it];
-    const int n = int(bi.end - bi.begin);
-    if (n > kSortMax) {
-      for (int i = threadIdx.x; i < n; i += blockDim.x) perm[bi.begin + i] = bi.begin + uint32_t(i);
-      continue;
-    }
-    int P = 1;
-    while (P < n) P <<= 1;
-    for (int i = threadIdx.x; i < P; i += blockDim.x)
-      key[i] = i < n ? ((bstamp[bi.begin + i] >> 32) << 32) | uint64_t(i) : ~uint64_t(0);
-    __syncthreads();
-    for (int k = 2; k <= P; k <<= 1) {
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int i = threadIdx.x; i < P; i += blockDim.x) {
-          const int l = i ^ j;
-          if (l > i) {
-            const uint64_t a = key[i], b = key[l];
-            if ((a > b) == ((i & k) == 0)) {
-              key[i] = b;
-              key[l] = a;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) perm[bi.begin + i] = bi.begin + uint32_t(key[i] & 0xFFFFFFFFu);
-    __syncthreads();
-  }
-}
-
-__global__ __launch_bounds__(256) void plan_permute(const uint32_t* perm, int64_t n, const uint64_t* bs_in,
-                                                    const uint32_t* rec_in, uint64_t* bs_out, uint32_t* rec_out) {
-  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x) {
-    const uint32_t q = perm[k];
-    bs_out[k] = bs_in[q];
-    rec_out[k] = rec_in[q];
-  }
-}
-
-int batched_sort_items(const BinItem* items, uint32_t nitems, int64_t n, uint64_t* bstamp, uint32_t* recs,
-                       uint32_t* perm, uint64_t* bs_tmp, uint32_t* rec_tmp, hipStream_t st) {
-  if (nitems == 0 || n == 0) return DDF_OK;
-  const unsigned nb = nitems < 65536u ? nitems : 65536u;
-  hipLaunchKernelGGL(plan_sort_items, dim3(nb), dim3(256), 0, st, items, nitems, bstamp, perm);
-  if (hipMemcpyAsync(bs_tmp, bstamp, size_t(n) * 8, hipMemcpyDeviceToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(rec_tmp, recs, size_t(n) * 4, hipMemcpyDeviceToDevice, st) != hipSuccess)
-    return DDF_EHIP;
-  int64_t g = (n + 255) / 256;
-  if (g > 65536) g = 65536;
-  hipLaunchKernelGGL(plan_permute, dim3(unsigned(g)), dim3(256), 0, st, perm, n, bs_tmp, rec_tmp, bstamp, recs);
-  return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
-}
 
 int batched_stamp_records(const BatchParams& p, const uint64_t* place, const uint32_t* recs, int64_t n,
                           uint64_t* bstamp, hipStream_t st) {

@@ -168,6 +168,9 @@ struct BatchParams {
   // tiles
   int tile, halo, ntiles_x, ntiles_y;
   uint32_t nitems_b;
+  // blocks of the pass's BDA map: the per-(block, facet) values between the passes are stored
+  // facet-major, [facet][block][nstokes]
+  int64_t nblk;
 };
 
 // Staged visibility entry of gridding pass A (floats): weight, flag, row, pad; READ complex
@@ -212,10 +215,6 @@ int batched_tile_wgs(const BatchParams& pg, const BatchParams& pd, unsigned out[
 // stamp records (tile offset | support << 16 | CF offset << 32) of the binned records
 int batched_stamp_records(const BatchParams& p, const uint64_t* place, const uint32_t* recs, int64_t n,
                           uint64_t* bstamp, hipStream_t st);
-// records of every bin item reordered by CF sub-kernel (bstamp and recs permuted together; perm,
-// bs_tmp and rec_tmp are n-entry scratch buffers)
-int batched_sort_items(const BinItem* items, uint32_t nitems, int64_t n, uint64_t* bstamp, uint32_t* recs,
-                       uint32_t* perm, uint64_t* bs_tmp, uint32_t* rec_tmp, hipStream_t st);
 int batched_prepare_jones(const BatchParams& p, const float2* src, double2* dst, float2* dstf, int dd, hipStream_t st);
 
 }  // namespace ddf

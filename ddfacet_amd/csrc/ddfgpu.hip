@@ -1041,20 +1041,6 @@ void fill_batch(BatchParams& p, const ddf_grid_desc* g, const ddf_facet* f0, con
 }
 
 // Count -> scan -> fill of the valid records, then the work items (largest first).
-// Records of each bin item grouped by CF sub-kernel (batched_sort_items), with stream-ordered
-// scratch.
-int sort_items(const BinItem* items, uint32_t ni, int64_t nr, uint64_t* bstamp, uint32_t* recs, hipStream_t st) {
-  if (!ni || !nr) return DDF_OK;
-  Temp t(st);
-  uint32_t* perm = static_cast<uint32_t*>(t.get(size_t(nr) * 4));
-  uint64_t* bs_tmp = static_cast<uint64_t*>(t.get(size_t(nr) * 8));
-  uint32_t* rec_tmp = static_cast<uint32_t*>(t.get(size_t(nr) * 4));
-  if (!perm || !bs_tmp || !rec_tmp) return fail(DDF_ENOMEM, "plan scratch allocation failed");
-  if (batched_sort_items(items, ni, nr, bstamp, recs, perm, bs_tmp, rec_tmp, st))
-    return fail(DDF_EHIP, "plan_sort_items launch failed");
-  return DDF_OK;
-}
-
 int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place, int chunk, uint32_t** recs_out,
                BinItem** items_out, uint32_t* nitems, int64_t* nrec, hipStream_t st, uint32_t** tile_first_out = nullptr,
                uint32_t** tile_count_out = nullptr) {
@@ -1299,6 +1285,8 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   }
   fill_vis(P->vg, vgrid, sg);
   fill_vis(P->vd, vdegrid, sd);
+  P->pg.nblk = vgrid->nblocks;
+  P->pd.nblk = vdegrid->nblocks;
   // first frequency and step of equidistant channels (the anchors of ChanPhasor)
   for (VisP* pv : {&P->vg, &P->vd}) {
     if (!pv->equi) continue;
@@ -1327,8 +1315,7 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   P->bstamp_g = P->alloc<uint64_t>(size_t(nr));
   if (!P->bstamp_g || batched_stamp_records(P->pg, P->place_g, P->recs_g, nr, P->bstamp_g, st))
     return bail(fail(DDF_ENOMEM, "plan allocation failed"));
-  rc = sort_items(P->items_g, ni, nr, P->bstamp_g, P->recs_g, st);
-  if (rc) return bail(rc);
+
   {
     // tile slots of the gridding items (plain stores + a deterministic reduction instead of
     // global atomics); without the memory for them the tiles are flushed with atomics
@@ -1348,8 +1335,7 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   P->bstamp_d = P->alloc<uint64_t>(size_t(nr));
   if (!P->bstamp_d || batched_stamp_records(P->pd, P->place_d, P->recs_d, nr, P->bstamp_d, st))
     return bail(fail(DDF_ENOMEM, "plan allocation failed"));
-  rc = sort_items(P->items_d, ni, nr, P->bstamp_d, P->recs_d, st);
-  if (rc) return bail(rc);
+
   // pass A layout
   const int64_t bpi = o.blocks_per_item > 0 ? o.blocks_per_item : 128;
   P->pg.blocks_per_item = bpi;
