@@ -858,11 +858,11 @@ __global__ __launch_bounds__(64) void grid_pass_b(BatchParams p, const uint64_t*
     }
     if (slots) {
       // the whole tile into this item's slot with plain stores (grid_slot_reduce adds the slots)
-      float4* dst = reinterpret_cast<float4*>(slots + size_t(it) * npg * WW);
+      float4* dst = reinterpret_cast<float4*>(slots + size_t(bi.slot) * npg * WW);
       const float4* src = reinterpret_cast<const float4*>(tile);
       const int n4 = (npg * WW) / 2;
       for (int i = lane; i < n4; i += 64) dst[i] = src[i];
-      if ((npg * WW) & 1 && lane == 0) slots[size_t(it) * npg * WW + npg * WW - 1] = tile[npg * WW - 1];
+      if ((npg * WW) & 1 && lane == 0) slots[size_t(bi.slot) * npg * WW + npg * WW - 1] = tile[npg * WW - 1];
     } else {
       DDF_GLOBAL float* grid = gptr_w(reinterpret_cast<float*>(fd.grid));
       for (int i = lane; i < npg * WW; i += 64) {

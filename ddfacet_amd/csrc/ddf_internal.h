@@ -139,6 +139,7 @@ struct FacetDev {
 struct BinItem {
   uint32_t begin, end;     // record range in the binned record list
   int16_t facet, band, tx, ty;
+  uint32_t slot;           // tile slot (the slots of one tile are consecutive)
 };
 
 struct BatchParams {

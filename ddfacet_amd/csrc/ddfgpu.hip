@@ -1061,6 +1061,12 @@ int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place
   int64_t tot = 0;
   std::vector<BinItem> items;
   std::vector<uint32_t> tfirst(static_cast<size_t>(ntile_all)), tcount(static_cast<size_t>(ntile_all));
+  // A tile's sub-bins (CF planes) are split into ngrp groups of consecutive planes, each chunked
+  // into items.  Slots are numbered tile by tile (a tile's slots consecutive, for
+  // grid_slot_reduce); the launch order is (facet, band, plane group, tile), so that the
+  // workgroups running at one time read the CF planes of one group and those stay in L2.
+  const int64_t ngrp = nsub < 4 ? nsub : 4;
+  std::vector<int64_t> order_key;
   for (int64_t i = 0; i < ntile_all; ++i) {
     tfirst[size_t(i)] = uint32_t(items.size());
     int64_t c = 0;
@@ -1074,23 +1080,40 @@ int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place
       const int ty = int(rem % p.ntiles_y); rem /= p.ntiles_y;
       const int band = int(rem % p.nband); rem /= p.nband;
       const int f = int(rem);
-      for (int64_t b = 0; b < c; b += chunk) {
-        BinItem it;
-        it.begin = uint32_t(tot + b);
-        it.end = uint32_t(tot + (b + chunk < c ? b + chunk : c));
-        it.facet = int16_t(f);
-        it.band = int16_t(band);
-        it.tx = int16_t(tx);
-        it.ty = int16_t(ty);
-        items.push_back(it);
+      const int64_t tile_in_fb = int64_t(ty) * p.ntiles_x + tx;
+      int64_t g0 = 0;
+      for (int64_t g = 0; g < ngrp; ++g) {
+        const int64_t q1 = (g + 1) * nsub / ngrp;
+        int64_t cg = 0;
+        for (int64_t q = g * nsub / ngrp; q < q1; ++q) cg += hc[size_t(i * nsub + q)];
+        for (int64_t b = 0; b < cg; b += chunk) {
+          BinItem it;
+          it.begin = uint32_t(tot + g0 + b);
+          it.end = uint32_t(tot + g0 + (b + chunk < cg ? b + chunk : cg));
+          it.facet = int16_t(f);
+          it.band = int16_t(band);
+          it.tx = int16_t(tx);
+          it.ty = int16_t(ty);
+          it.slot = uint32_t(items.size());
+          items.push_back(it);
+          order_key.push_back(((int64_t(f) * p.nband + band) * ngrp + g) * (int64_t(p.ntiles_x) * p.ntiles_y) +
+                              tile_in_fb);
+        }
+        g0 += cg;
       }
     }
     tcount[size_t(i)] = uint32_t(items.size()) - tfirst[size_t(i)];
     tot += c;
   }
   if (tot >= (int64_t(1) << 32)) return fail(DDF_EINVAL, "too many (block, facet) records for one plan");
-  // items stay in facet-major order: workgroups that run at the same time work on few facets, so
-  // the CF planes they read stay in cache
+  {
+    std::vector<size_t> idx(items.size());
+    for (size_t k = 0; k < idx.size(); ++k) idx[k] = k;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return order_key[a] < order_key[b]; });
+    std::vector<BinItem> ordered(items.size());
+    for (size_t k = 0; k < idx.size(); ++k) ordered[k] = items[idx[k]];
+    items.swap(ordered);
+  }
   HIP_TRY(hipMemcpyAsync(bstart, hs.data(), size_t(nbins) * 8, hipMemcpyHostToDevice, st));
   uint32_t* recs = P->alloc<uint32_t>(size_t(tot));
   BinItem* d_items = P->alloc<BinItem>(items.size());
