@@ -53,6 +53,21 @@ def _jones_bytes(P):
     return int(np.asarray(P.jones["kms"]).size) * 8
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "batched_cycle_81facet_hbm_traffic.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the 81-facet cycle, from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a copy-kernel calibration); None when absent."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        d = json.loads(block)
+        return int(d["bytes_per_launch"][kernel]["traffic_bytes"])
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def cycle_bytes(P, nf):
     """Bytes of one major cycle of the per-facet algorithm (each facet a pass over the
     visibilities): per visibility-facet pair 40 (grid in) + 64 (degrid read + write) + 48 / C
@@ -444,6 +459,7 @@ def main():
         abytes = plain_bytes(P, "grid" if dom == "grid_plain" else "degrid")
         extra = {}
     ach = abytes / (launch_ms * 1e-3)
+    traffic = measured_traffic(dom) if (a.mode == "batched" and not a.facets) else None
     cb = cycle_bytes(P, nf)
     graph_ms = None
     if a.graph and a.mode == "batched":
@@ -462,7 +478,7 @@ def main():
         "dtype": "c64 (fp32 grids, fp64 phase base)", "data": "synthetic",
         "config": cfg,
         "roofline": dict({"bound": "hbm", "kernel": dom, "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9,
-                          "unit": "GB/s", "frac": ach / HBM_PEAK, "traffic": None, "launch_ms": launch_ms,
+                          "unit": "GB/s", "frac": ach / HBM_PEAK, "traffic": traffic, "launch_ms": launch_ms,
                           "algorithmic_bytes_per_launch": abytes,
                           "cycle": {"bytes": cb, "ms": ms, "achieved": cb / (ms * 1e-3) / 1e9,
                                     "frac": cb / (ms * 1e-3) / HBM_PEAK}}, **extra),
