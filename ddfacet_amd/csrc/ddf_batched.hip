@@ -925,46 +925,21 @@ __global__ __launch_bounds__(256) void grid_slot_reduce(BatchParams p, const flo
 }
 
 // ---- cycle: degrid pass A (LDS-staged gather) --------------------------------------------------
-// The model tile of a bin item is staged in LDS (separate real / imaginary planes); each record's
-// stamp is gathered by one wavefront (next record's CF values loaded ahead): FP32 products
-// (grid x CF) summed in FP64, per lane and then across the wavefront (degridder.h:202-212 sums
-// the FP32 products in double).
-template <typename ACC>
-__device__ __forceinline__ void stamp_gather(const float* tile, int WW, int ns, const Stamp& s, const StampCells& sc,
-                                             ACC* sr, ACC* si) {
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    if (64 * r >= s.S2) break;
-    if (sc.off[r] < 0) continue;
-    const float* t = tile + s.base + sc.off[r];
-#pragma unroll
-    for (int ip = 0; ip < 4; ++ip) {
-      if (ip < ns) {
-        const float mr = t[(2 * ip) * WW], mi = t[(2 * ip + 1) * WW];
-        sr[ip] += ACC(mr * sc.w[r].x - mi * sc.w[r].y);
-        si[ip] += ACC(mr * sc.w[r].y + mi * sc.w[r].x);
-      }
-    }
-  }
-}
-__device__ __forceinline__ double wave_sum_acc(double x) { return wave_total(x); }
-__device__ __forceinline__ float wave_sum_acc(float x) { return wave_total_f32(x); }
-
-// ACC: the accumulator of the stamp sums (double as the reference; float in the mixed and FP32
-// precisions, where the FP32 products of one stamp are summed in FP32 per lane and across the
-// wavefront)
+// The model tile of a bin item is staged in LDS (complex64, 16-byte aligned); lanes are records:
+// each lane gathers its own record's S x S stamp, row by row, with FP32 products (grid x CF)
+// summed in ACC (degridder.h:202-212 sums the FP32 products in double; ACC = double in the FP64
+// precision, float in the mixed and FP32 precisions).  No cross-lane work: the records of a bin
+// share their CF plane, so the lanes of a wavefront mostly share S and run in step.
 // ONE: a single Stokes plane (nstokes == 1), resolved at compile time.
 template <typename ACC, bool ONE>
 __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64_t* bstamp, float2* M,
                                                      const BinItem* items, const uint32_t* recs, uint32_t* queue) {
-  extern __shared__ float tile[];  // [nstokes][2][W * W]
+  extern __shared__ __attribute__((aligned(16))) float tile_f[];
+  float2* tile = reinterpret_cast<float2*>(tile_f);  // [nstokes][W * W]
   __shared__ uint32_t s_item;
   const int W = p.tile + 2 * p.halo;
   const int WW = W * W;
   const int ns = ONE ? 1 : p.nstokes;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int nwave = blockDim.x >> 6;
   while (true) {
     if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
     __syncthreads();
@@ -982,63 +957,50 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
       float2 m = make_float2(0.f, 0.f);
       if (gx >= 0 && gy >= 0 && gx < p.nx && gy < p.ny)
         m = ld2(model + ((size_t(bi.band) * p.npol + ip) * p.ny + gy) * p.nx + gx);
-      tile[2 * ip * WW + rem] = m.x;
-      tile[(2 * ip + 1) * WW + rem] = m.y;
+      tile[i] = m;
     }
     __syncthreads();
-    for (uint32_t k0 = bi.begin + 64u * wave; k0 < bi.end; k0 += 64u * nwave) {
-      const uint32_t k = k0 + lane;
-      const bool ok = k < bi.end;
-      const uint64_t my_rec = ok ? bstamp[k] : 0;
-      const uint32_t my_t = ok ? recs[k] : 0;
-      float2 res[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
-      const int cnt = int(min(64u, bi.end - k0));
-      // three records in flight, buffers rotated statically (as in grid_pass_b)
-      Stamp sA = stamp_none(), sB = stamp_none(), sC = stamp_none();
-      StampCells cA, cB, cC;
-      auto load = [&](int j, Stamp& st_, StampCells& c) {
-        if (j < cnt) {
-          st_ = stamp_of(lane_u64(my_rec, j), cf);
-          fetch_cells(st_, lane, W, c);
-        }
-      };
-      auto gather = [&](int j, const Stamp& st_, const StampCells& c) {
-        if (j >= cnt) return;
-        ACC sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
-        stamp_gather(tile, WW, ns, st_, c, sr, si);
-        for (int c0 = 256; c0 < st_.S2; c0 += 256) {
-          StampCells xc;
+    for (uint32_t k = bi.begin + threadIdx.x; k < bi.end; k += blockDim.x) {
+      const uint64_t rec = bstamp[k];
+      const uint32_t t = recs[k];
+      const int S = int((rec >> 16) & 0x1F);
+      const int S2 = S * S;
+      const DDF_GLOBAL float2* cp = cf + uint32_t(rec >> 32);
+      ACC sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
+      int col = 0, toff = int(rec & 0xFFFF);  // column and tile offset of the next cell
+      for (int c = 0; c < S2; c += 4) {
+        float2 w[4];
+        int off[4];
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int c1 = c0 + 64 * r + lane;
-            xc.off[r] = stamp_cell(st_, c1, W);
-            xc.w[r] = ld2(st_.cfp + (c1 < st_.S2 ? c1 : 0));
-          }
-          stamp_gather(tile, WW, ns, st_, xc, sr, si);
-        }
-#pragma unroll
-        for (int ip = 0; ip < 4; ++ip) {
-          if (ip < ns) {
-            const ACC a = wave_sum_acc(sr[ip]), b = wave_sum_acc(si[ip]);
-            if (lane == j) res[ip] = make_float2(float(a), float(b));
+        for (int j = 0; j < 4; ++j) {
+          const bool in = c + j < S2;
+          w[j] = in ? ld2(cp + c + j) : make_float2(0.f, 0.f);
+          off[j] = in ? toff : -1;
+          if (++col == S) {
+            col = 0;
+            toff += W - S + 1;
+          } else {
+            ++toff;
           }
         }
-      };
-      load(0, sA, cA);
-      load(1, sB, cB);
-      for (int j = 0; j < cnt; j += 3) {
-        load(j + 2, sC, cC);
-        gather(j, sA, cA);
-        load(j + 3, sA, cA);
-        gather(j + 1, sB, cB);
-        load(j + 4, sB, cB);
-        gather(j + 2, sC, cC);
-      }
-      if (ok) {
 #pragma unroll
-        for (int ip = 0; ip < 4; ++ip)
-          if (ip < ns) M[(size_t(bi.facet) * p.nblk + my_t / uint32_t(p.nf)) * ns + ip] = res[ip];
+        for (int j = 0; j < 4; ++j) {
+          if (off[j] < 0) continue;  // past the stamp (last group of 4)
+#pragma unroll
+          for (int ip = 0; ip < 4; ++ip) {
+            if (ip < ns) {
+              const float2 m = tile[ip * WW + off[j]];
+              sr[ip] += ACC(m.x * w[j].x - m.y * w[j].y);
+              si[ip] += ACC(m.x * w[j].y + m.y * w[j].x);
+            }
+          }
+        }
       }
+      // value of record t = block * nf + facet, stored at [facet][block]
+      float2* out = M + (size_t(bi.facet) * p.nblk + t / uint32_t(p.nf)) * ns;
+#pragma unroll
+      for (int ip = 0; ip < 4; ++ip)
+        if (ip < ns) out[ip] = make_float2(float(sr[ip]), float(si[ip]));
     }
     __syncthreads();
   }
