@@ -345,6 +345,8 @@ def main():
                          "autograd25: 20 chi^2 gradient-descent steps through the differentiable "
                          "degridder on 25 facets x 4 bands")
     ap.add_argument("--tile", type=int, default=0, help="uv tile edge of the batched plan (0: the plan default)")
+    ap.add_argument("--group", type=int, default=0,
+                    help="facets per pass-A lane group of the batched plan (0: the plan default)")
     ap.add_argument("--graph", action="store_true",
                     help="also time the batched cycle replayed from a HIP graph")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -398,7 +400,7 @@ def main():
 
     stream = torch.cuda.current_stream()
     if a.mode == "batched":
-        fs.plan_batched(tile=a.tile, stream=stream)
+        fs.plan_batched(tile=a.tile, group=a.group, stream=stream)
         fs.plan_timing(True)
         info = fs.plan_info()
         log("plan: %s" % info)

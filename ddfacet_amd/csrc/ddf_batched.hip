@@ -264,7 +264,7 @@ __global__ void reset_counter(uint32_t* c, int n) {
 // JMODE: 0 no Jones, 1 per-visibility Jones (applied once per run of one Jones key), 2 per-block
 // Jones, 3 per-visibility Jones with ScaleJones (applied per visibility).
 template <int READ, int JMODE, int PREC>
-__global__ __launch_bounds__(256) void grid_pass_a(BatchParams p, const VisP v, const uint64_t* place, float2* V,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void grid_pass_a(BatchParams p, const VisP v, const uint64_t* place, float2* V,
                                                    uint32_t* queue) {
   // LDS: per-facet sums of every facet [nf][nband*npol + 2*nband], flushed once at the end; the
   // per-channel Jones sums of the current facet group [G][2][nchan] (when they fit), flushed when
@@ -1022,11 +1022,11 @@ struct RunInfo {
 };
 
 template <int NCORR, int JMODE, int PREC>
-__global__ __launch_bounds__(256) void degrid_pass_b(BatchParams p, const VisP v, const float2* M,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void degrid_pass_b(BatchParams p, const VisP v, const float2* M,
                                                      const uint64_t* vrc, const uint32_t* vblk,
                                                      const uint32_t* vrun, const uint32_t* run_first, int64_t nvl) {
   using RT = typename std::conditional<PREC == 1, double, float>::type;
-  constexpr int FB = PREC == 1 ? 4 : 8;
+  constexpr int FB = 4;
   __shared__ cx<RT> sY[FB][256][4];
   __shared__ uint8_t sNZ[FB][256];
   __shared__ RunInfo sRun[256];
