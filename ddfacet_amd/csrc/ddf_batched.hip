@@ -194,7 +194,22 @@ __global__ __launch_bounds__(256) void plan_stamp_records(BatchParams p, const u
     const int x0 = (u.locx / p.tile) * p.tile - p.halo, y0 = (u.locy / p.tile) * p.tile - p.halo;
     const int base = (u.locy - sup - y0) * W + (u.locx - sup - x0);
     const int64_t cfoff = gptr(gload(p.facets + f).cf_offset)[u.plane] + int64_t(u.io * p.overs + u.jo) * S * S;
-    bstamp[k] = uint64_t(uint32_t(base) & 0xFFFF) | (uint64_t(S & 0x1F) << 16) | (uint64_t(uint32_t(cfoff)) << 32);
+    const uint32_t sub = uint32_t(u.io * p.overs + u.jo);
+    bstamp[k] = uint64_t(uint32_t(base) & 0xFFFF) | (uint64_t(S & 0x1F) << 16) | (uint64_t(sub & 0x7FF) << 21) |
+                (uint64_t(uint32_t(cfoff)) << 32);
+  }
+}
+
+__global__ __launch_bounds__(256) void plan_cf_transpose(const float2* in, float2* out, const int64_t* off,
+                                                         const int32_t* support, int nw, int overs) {
+  const int pl = blockIdx.y;
+  const int S = support[pl < nw ? pl : pl - nw];
+  const int S2 = S * S, O2 = overs * overs;
+  const int64_t o = off[pl];
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < int64_t(O2) * S2;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int k = int(i / S2), c = int(i - int64_t(k) * S2);
+    out[o + int64_t(c) * O2 + k] = in[o + i];
   }
 }
 
@@ -947,7 +962,7 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
     if (it >= p.nitems_b) break;
     const BinItem bi = items[it];
     const FacetDev fd = gload(p.facets + bi.facet);
-    const DDF_GLOBAL float2* cf = gptr(fd.cf);
+    const DDF_GLOBAL float2* cft = gptr(fd.cft);
     const int x0 = bi.tx * p.tile - p.halo, y0 = bi.ty * p.tile - p.halo;
     const DDF_GLOBAL float2* model = gptr(reinterpret_cast<const float2*>(fd.model));
     for (int i = threadIdx.x; i < ns * WW; i += blockDim.x) {
@@ -965,7 +980,11 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
       const uint32_t t = recs[k];
       const int S = int((rec >> 16) & 0x1F);
       const int S2 = S * S;
-      const DDF_GLOBAL float2* cp = cf + uint32_t(rec >> 32);
+      // cell c of sub-kernel `sub` at plane offset + c * overs^2 + sub of the transposed planes:
+      // the lanes of a wavefront (records of one plane) read neighbouring words
+      const int sub = int((rec >> 21) & 0x7FF);
+      const int O2 = p.overs * p.overs;
+      const DDF_GLOBAL float2* cp = cft + (int64_t(uint32_t(rec >> 32)) - int64_t(sub) * S2 + sub);
       ACC sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
       int col = 0, toff = int(rec & 0xFFFF);  // column and tile offset of the next cell
       for (int c = 0; c < S2; c += 4) {
@@ -974,7 +993,7 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const bool in = c + j < S2;
-          w[j] = in ? ld2(cp + c + j) : make_float2(0.f, 0.f);
+          w[j] = in ? ld2(cp + int64_t(c + j) * O2) : make_float2(0.f, 0.f);
           off[j] = in ? toff : -1;
           if (++col == S) {
             col = 0;
@@ -1576,6 +1595,17 @@ int batched_degrid_b_adjoint(const BatchParams& p, const VisP& v, const uint64_t
 }  // namespace ddf
 
 namespace ddf {
+
+int batched_cf_transpose(const float2* in, float2* out, const int64_t* off, const int32_t* support, int nw, int overs,
+                         int max_support, hipStream_t st) {
+  const int64_t n = int64_t(overs) * overs * max_support * max_support;
+  int64_t nb = (n + 255) / 256;
+  if (nb > 1024) nb = 1024;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(plan_cf_transpose, dim3(unsigned(nb), unsigned(2 * nw)), dim3(256), 0, st, in, out, off, support, nw,
+                     overs);
+  return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
+}
 
 int batched_stamp_records(const BatchParams& p, const uint64_t* place, const uint32_t* recs, int64_t n,
                           uint64_t* bstamp, hipStream_t st) {

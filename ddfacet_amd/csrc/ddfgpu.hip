@@ -1242,6 +1242,8 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
       return bail(fail(DDF_EINVAL, "uv tile too large for LDS at this support and polarisation count"));
     // stamp records hold the tile offset in 16 bits and the support in 5
     if (W > 255 || maxsup > 31) return bail(fail(DDF_EINVAL, "uv tile or CF support too large for the stamp records"));
+    // ... and the oversampled sub-kernel index in 11 bits
+    if (int64_t(c0.overs) * c0.overs > 0x7FF) return bail(fail(DDF_EINVAL, "CF oversampling too large for the stamp records"));
   }
   int G = o.group;
   if (G <= 0) {
@@ -1250,6 +1252,9 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   }
   if (G > 64) G = 64;  // gridding pass A aligns its slots (G lanes each) to wavefronts
   // facet descriptors
+  P->d_support = P->alloc<int32_t>(sup0.size());
+  if (!P->d_support || hipMemcpyAsync(P->d_support, sup0.data(), sup0.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+    return bail(fail(DDF_EHIP, "plan upload failed"));
   std::vector<FacetDev> hf(static_cast<size_t>(nfacets));
   for (int i = 0; i < nfacets; ++i) {
     const ddf_facet& f = facets[i];
@@ -1257,6 +1262,24 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
     d.grid = f.grid;
     d.model = models[i];
     d.cf = reinterpret_cast<const float2*>(f.cf.data);
+    {
+      // transposed copy of the facet's CF planes for degrid pass A (same offsets)
+      const int nw = f.cf.nw, ov = f.cf.overs;
+      std::vector<int64_t> off(size_t(2 * nw));
+      if (hipMemcpyAsync(off.data(), f.cf.offset, off.size() * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return bail(fail(DDF_EHIP, "reading the CF plane offsets failed"));
+      int64_t total = 0;
+      for (int pl = 0; pl < 2 * nw; ++pl) {
+        const int64_t S = sup0[size_t(pl < nw ? pl : pl - nw)];
+        total = std::max(total, off[size_t(pl)] + int64_t(ov) * ov * S * S);
+      }
+      float2* t = P->alloc<float2>(size_t(total));
+      if (!t) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
+      if (batched_cf_transpose(d.cf, t, f.cf.offset, P->d_support, nw, ov, maxsup, st))
+        return bail(fail(DDF_EHIP, "CF transpose launch failed"));
+      d.cft = t;
+    }
     d.cf_offset = f.cf.offset;
     d.cu = f.cu;
     d.cv = f.cv;
@@ -1272,11 +1295,9 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
     d.sum_jones_chan = f.jones.sum_jones_chan;
   }
   P->d_facets = P->alloc<FacetDev>(hf.size());
-  P->d_support = P->alloc<int32_t>(sup0.size());
   P->queues = P->alloc<uint32_t>(4);
-  if (!P->d_facets || !P->d_support || !P->queues) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
-  if (hipMemcpyAsync(P->d_facets, hf.data(), hf.size() * sizeof(FacetDev), hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(P->d_support, sup0.data(), sup0.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+  if (!P->d_facets || !P->queues) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
+  if (hipMemcpyAsync(P->d_facets, hf.data(), hf.size() * sizeof(FacetDev), hipMemcpyHostToDevice, st) != hipSuccess)
     return bail(fail(DDF_EHIP, "plan upload failed"));
   for (BatchParams* bp : {&P->pg, &P->pd}) {
     bp->facets = P->d_facets;
