@@ -193,6 +193,36 @@ def cpu_baseline(P, target_s=8.0, max_workers=16):
                        % (len(facets), facets, stride, nv, t, nf))
 
 
+def error_vs_fp64(fs, vis0, tile, group, stream):
+    """Relative L2 distance of the timed (mixed-precision) cycle's facet grids and predicted
+    visibilities to the same cycle run by an FP64-precision plan of the same chunk (untimed)."""
+    import torch
+
+    def cycle():
+        fs.zero_grids()
+        fs.t["vis"].copy_(vis0)
+        fs.grid_batched(stream=stream)
+        # degridding subtracts the prediction: from zeros, so that the FP32 rounding of the
+        # observed visibilities does not enter the comparison
+        fs.t["vis"].zero_()
+        fs.degrid_batched(stream=stream)
+        torch.cuda.synchronize()
+        out = (fs.grids.clone(), (-fs.t["vis"]).clone())
+        fs.t["vis"].copy_(vis0)
+        return out
+
+    def rel(x, y):
+        return float(torch.linalg.vector_norm((x - y).to(torch.complex128)) /
+                     torch.linalg.vector_norm(y.to(torch.complex128)))
+
+    g_mixed, p_mixed = cycle()
+    fs.plan_batched(tile=tile, group=group, precision="fp64", stream=stream)
+    g64, p64 = cycle()
+    fs.destroy_plan()
+    return {"grids": rel(g_mixed, g64), "predicted_vis": rel(p_mixed, p64),
+            "note": "the timed cycle's arithmetic (mixed precision) against the FP64-precision plan"}
+
+
 def time_graph(fs, steps):
     """Mean ms of one batched cycle replayed from a HIP graph (capture excluded)."""
     import torch
@@ -349,6 +379,8 @@ def main():
                     help="facets per pass-A lane group of the batched plan (0: the plan default)")
     ap.add_argument("--graph", action="store_true",
                     help="also time the batched cycle replayed from a HIP graph")
+    ap.add_argument("--no-fp64-check", action="store_true",
+                    help="skip the untimed FP64-precision cycle that measures the error of the timed one")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only an elementwise copy of 1 GiB (reads 1 GiB, writes 1 GiB), to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiler run")
@@ -467,6 +499,9 @@ def main():
     if a.graph and a.mode == "batched":
         graph_ms = time_graph(fs, a.steps)
         fs.t["vis"].copy_(vis0)
+    err64 = None
+    if a.mode == "batched" and not a.no_fp64_check:
+        err64 = error_vs_fp64(fs, vis0, a.tile, a.group, stream)
     cfg = {"workload": "81-facet MeerKAT-like: 9x9 facets (455 -> 735 px at 1.5\"), 64 antennas, "
                        "600 x 8 s, 64 ch 0.9-1.67 GHz, 4 corr, per-visibility DD Jones, default BDA (0.02)",
            "rows": R, "channels": C, "nvis": nvis, "facets": nf,
@@ -474,7 +509,7 @@ def main():
     if a.mode == "batched":
         cfg.update(grid_records=info["grid_records"], degrid_records=info["degrid_records"])
     out = {
-        "metric": "vis/sec grid+degrid, 81-facet MeerKAT-like",
+        "metric": "vis/sec grid+degrid, 81-facet MeerKAT-like; % HBM roofline; rel err vs FP64",
         "value": value, "unit": "vis/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64 (fp32 grids, fp64 phase base)", "data": "synthetic",
@@ -488,6 +523,8 @@ def main():
     }
     if graph_ms is not None:
         out["graph_ms_per_step"] = graph_ms
+    if err64 is not None:
+        out["rel_err_vs_fp64"] = err64
     print(json.dumps(out))
 
 

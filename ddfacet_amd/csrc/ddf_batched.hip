@@ -943,7 +943,7 @@ __global__ __launch_bounds__(256) void grid_slot_reduce(BatchParams p, const flo
 // The model tile of a bin item is staged in LDS (complex64, 16-byte aligned); lanes are records:
 // each lane gathers its own record's S x S stamp, row by row, with FP32 products (grid x CF)
 // summed in ACC (degridder.h:202-212 sums the FP32 products in double; ACC = double in the FP64
-// precision, float in the mixed and FP32 precisions).  No cross-lane work: the records of a bin
+// and mixed precisions, float in the FP32 precision).  No cross-lane work: the records of a bin
 // share their CF plane, so the lanes of a wavefront mostly share S and run in step.
 // ONE: a single Stokes plane (nstokes == 1), resolved at compile time.
 template <typename ACC, bool ONE>
@@ -1447,10 +1447,10 @@ int batched_tile_wgs(const BatchParams& pg, const BatchParams& pd, unsigned out[
                     : persistent_wgs(grid_pass_b<false, false>, size_t(npg) * Wg * Wg * sizeof(float2), 0, 64);
   const size_t lds_d = size_t(pd.nstokes) * Wd * Wd * 2 * sizeof(float);
   if (pd.nstokes == 1)
-    out[1] = pd.prec == 1 ? persistent_wgs(degrid_pass_a<double, true>, lds_d, 0)
+    out[1] = pd.prec != 2 ? persistent_wgs(degrid_pass_a<double, true>, lds_d, 0)
                           : persistent_wgs(degrid_pass_a<float, true>, lds_d, 0);
   else
-    out[1] = pd.prec == 1 ? persistent_wgs(degrid_pass_a<double, false>, lds_d, 0)
+    out[1] = pd.prec != 2 ? persistent_wgs(degrid_pass_a<double, false>, lds_d, 0)
                           : persistent_wgs(degrid_pass_a<float, false>, lds_d, 0);
   out[2] = npd == 1 ? persistent_wgs(grid_pass_b<true, true>, size_t(pd.nstokes) * Wd * Wd * sizeof(float2), 0, 64)
                     : persistent_wgs(grid_pass_b<true, false>, size_t(pd.nstokes) * Wd * Wd * sizeof(float2), 0, 64);
@@ -1509,7 +1509,7 @@ int batched_degrid_a(const BatchParams& p, const uint64_t* place, float2* M, con
                      const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st) {
   const int W = p.tile + 2 * p.halo;
   size_t lds = size_t(p.nstokes) * W * W * 2 * sizeof(float);
-  if (p.prec == 1) {
+  if (p.prec != 2) {  // FP64 stamp sums (as the reference) except in the FP32 precision
     if (p.nstokes == 1) launch_degrid_a<double, true>(p, place, M, items, recs, queue, nwg, lds, st);
     else launch_degrid_a<double, false>(p, place, M, items, recs, queue, nwg, lds, st);
   } else {
