@@ -180,7 +180,8 @@ typedef struct ddf_plan_opts {
   int32_t group;           /* facets per lane group in gridding (0: auto, <= 64)                */
   int32_t precision;       /* 0 mixed: FP64 uvw rotation and per-row phase base, FP32 sincos and
                               FP32 phasor advance along channels (re-anchored every 16), FP32
-                              block sums / Jones / stamp sums, FP64 weight sums;
+                              run sums / Jones products / Stokes maps, FP64 stamp, facet and
+                              weight sums;
                               1 FP64 arithmetic throughout (as the reference);
                               2 FP32 throughout (phase formed and reduced in FP32)              */
   int32_t workgroups;      /* persistent workgroups per launch (0: 4 per compute unit for the
