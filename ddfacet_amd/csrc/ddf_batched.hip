@@ -321,6 +321,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
   float* stg = s_stage + size_t(active ? slot : 0) * K * kStride;
   const int npg = p.nstokes < p.npol ? p.nstokes : p.npol;
   __shared__ uint32_t s_item;
+  __shared__ uint32_t s_next;
   int cur_group = -1;
 
   auto flush_chan = [&](int group) {
@@ -337,7 +338,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
   };
 
   while (true) {
-    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
+    if (threadIdx.x == 0) {
+      s_item = atomicAdd(queue, 1u);
+      s_next = 0;
+    }
     __syncthreads();
     const uint32_t item = s_item;
     if (item >= p.nitems_a) break;
@@ -352,7 +356,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
     const int fc = f < p.nf ? f : p.nf - 1;
     const int64_t b0 = chunk * p.blocks_per_item;
     const int64_t b1 = min(b0 + p.blocks_per_item, v.nblocks);
-    for (int64_t b = b0 + slot; active && b < b1; b += slots) {
+    // the slots take the item's blocks one at a time from an LDS counter (blocks differ in size)
+    auto next_block = [&]() -> int64_t {
+      uint32_t kb_ = 0;
+      if (lf == 0) kb_ = atomicAdd(&s_next, 1u);
+      return b0 + int64_t(__shfl(int(kb_), sw * G, 64));
+    };
+    for (int64_t b = active ? next_block() : b1; active && b < b1; b = next_block()) {
       // every lane of the slot stages; lanes whose record is dropped (or past the last facet)
       // only stage
       const uint64_t rec = place[b * p.nf + fc];
@@ -614,34 +624,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
       }
-      if (!doit) continue;
-      if (JMODE == 1) {
-        if (any) flush();
-        ssw = dec2 * swt;
-        sj = dec2 * sjr;
-      }
-      if (JMODE == 2) {
-        M2 m{{acc[0], acc[1], acc[2], acc[3]}};
-        if (p.dopsf) m = mul(mul(jblk.J0, m), herm(jblk.J1));
-        m = mul(mul(herm(jblk.J0), m), jblk.J1);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[q] = m.a[q];
-        sj = ssw * BBblk;
-      }
-      // facet-major [f][block]: the tile pass reads one facet's records in block order
-      float2* out = V + (size_t(f) * v.nblocks + b) * p.nstokes;
-      for (int k = 0; k < p.nstokes; ++k) {
-        cd s = cd{0, 0};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) s = s + cd{p.sm[k][q][0], p.sm[k][q][1]} * acc[q];
-        out[k] = make_float2(float(s.re), float(s.im));
-      }
-      double* a = s_acc + f * per_f;
-      for (int ip = 0; ip < npg; ++ip) {
-        atomicAdd(&a[u.band * p.npol + ip], swt);
-        if (JMODE != 0) {
-          atomicAdd(&a[nbp + u.band], sj);
-          atomicAdd(&a[nbp + p.nband + u.band], ssw);
+      if (doit) {
+        if (JMODE == 1) {
+          if (any) flush();
+          ssw = dec2 * swt;
+          sj = dec2 * sjr;
+        }
+        if (JMODE == 2) {
+          M2 m{{acc[0], acc[1], acc[2], acc[3]}};
+          if (p.dopsf) m = mul(mul(jblk.J0, m), herm(jblk.J1));
+          m = mul(mul(herm(jblk.J0), m), jblk.J1);
+  #pragma unroll
+          for (int q = 0; q < 4; ++q) acc[q] = m.a[q];
+          sj = ssw * BBblk;
+        }
+        // facet-major [f][block]: the tile pass reads one facet's records in block order
+        float2* out = V + (size_t(f) * v.nblocks + b) * p.nstokes;
+        for (int k = 0; k < p.nstokes; ++k) {
+          cd s = cd{0, 0};
+  #pragma unroll
+          for (int q = 0; q < 4; ++q) s = s + cd{p.sm[k][q][0], p.sm[k][q][1]} * acc[q];
+          out[k] = make_float2(float(s.re), float(s.im));
+        }
+        double* a = s_acc + f * per_f;
+        for (int ip = 0; ip < npg; ++ip) {
+          atomicAdd(&a[u.band * p.npol + ip], swt);
+          if (JMODE != 0) {
+            atomicAdd(&a[nbp + u.band], sj);
+            atomicAdd(&a[nbp + p.nband + u.band], ssw);
+          }
         }
       }
     }
