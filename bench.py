@@ -193,7 +193,7 @@ def cpu_baseline(P, target_s=8.0, max_workers=16):
                        % (len(facets), facets, stride, nv, t, nf))
 
 
-def error_vs_fp64(fs, vis0, tile, group, stream):
+def error_vs_fp64(fs, vis0, tile, group, blocks_per_item, stream):
     """Relative L2 distance of the timed (mixed-precision) cycle's facet grids and predicted
     visibilities to the same cycle run by an FP64-precision plan of the same chunk (untimed)."""
     import torch
@@ -216,7 +216,7 @@ def error_vs_fp64(fs, vis0, tile, group, stream):
                      torch.linalg.vector_norm(y.to(torch.complex128)))
 
     g_mixed, p_mixed = cycle()
-    fs.plan_batched(tile=tile, group=group, precision="fp64", stream=stream)
+    fs.plan_batched(tile=tile, group=group, blocks_per_item=blocks_per_item, precision="fp64", stream=stream)
     g64, p64 = cycle()
     fs.destroy_plan()
     return {"grids": rel(g_mixed, g64), "predicted_vis": rel(p_mixed, p64),
@@ -375,6 +375,8 @@ def main():
                          "autograd25: 20 chi^2 gradient-descent steps through the differentiable "
                          "degridder on 25 facets x 4 bands")
     ap.add_argument("--tile", type=int, default=0, help="uv tile edge of the batched plan (0: the plan default)")
+    ap.add_argument("--blocks-per-item", type=int, default=0,
+                    help="BDA blocks per pass-A work item of the batched plan (0: the plan default)")
     ap.add_argument("--group", type=int, default=0,
                     help="facets per pass-A lane group of the batched plan (0: the plan default)")
     ap.add_argument("--graph", action="store_true",
@@ -432,7 +434,7 @@ def main():
 
     stream = torch.cuda.current_stream()
     if a.mode == "batched":
-        fs.plan_batched(tile=a.tile, group=a.group, stream=stream)
+        fs.plan_batched(tile=a.tile, group=a.group, blocks_per_item=a.blocks_per_item, stream=stream)
         fs.plan_timing(True)
         info = fs.plan_info()
         log("plan: %s" % info)
@@ -501,7 +503,7 @@ def main():
         fs.t["vis"].copy_(vis0)
     err64 = None
     if a.mode == "batched" and not a.no_fp64_check:
-        err64 = error_vs_fp64(fs, vis0, a.tile, a.group, stream)
+        err64 = error_vs_fp64(fs, vis0, a.tile, a.group, a.blocks_per_item, stream)
     cfg = {"workload": "81-facet MeerKAT-like: 9x9 facets (455 -> 735 px at 1.5\"), 64 antennas, "
                        "600 x 8 s, 64 ch 0.9-1.67 GHz, 4 corr, per-visibility DD Jones, default BDA (0.02)",
            "rows": R, "channels": C, "nvis": nvis, "facets": nf,
