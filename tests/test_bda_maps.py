@@ -1,5 +1,5 @@
-"""BDA block maps (ClassSmearMapping.py:83-135 format, :496-625 mode-1 builder) against
-hand-worked cases.
+"""BDA block maps (ClassSmearMapping.py:83-135 format, :496-625 mode-1 builder, :348-491 mode-2
+builder) against hand-worked cases.
 
 The reference builder is Python that cannot be imported here (it needs the process pool and its
 native helpers), so the restatement in ddfacet_amd/bda.py is pinned by cases whose blocks follow
@@ -139,3 +139,89 @@ def test_mode1_row_step_is_the_distance_to_the_next_row():
     # nch = 15 * 1e9 / (1e6 * 1e7) < 1 -> single channels
     want = [(c, c + 1, [0, 1, 2]) for c in range(8)] + [(c, c + 1, [3, 4, 5]) for c in range(8)]
     assert _blocks(m) == want
+
+
+# ---- mode 2 (GiveBlocksRowsListBL, ClassSmearMapping.py:348-491) ------------------------------------
+# With nu_max = C / (2 pi) the phase factor 2 pi nu_max / C is 1, so a row's phase step is
+# |du,dv| lmax + |dw| nmax and its channel span is nu_max dPhi / (ext dfreq) with
+# ext = |u,v| lmax + |w| nmax.
+NU1 = C / (2 * np.pi)
+
+
+def _mode2(uvw, dphi, lmax, nchan=8, dfreq=None, chan_map=None):
+    freqs = np.linspace(0.5 * NU1, NU1, nchan)
+    dfreq = NU1 / 100.0 if dfreq is None else dfreq
+    chan_map = np.zeros(nchan, np.int32) if chan_map is None else np.asarray(chan_map, np.int32)
+    ind = np.arange(uvw.shape[0])
+    sizes, flat = bda._blocks_bl_mode2(ind, uvw, dfreq, freqs, dphi, lmax, chan_map)
+    return _blocks(bda.pack({"0:1": (sizes, flat)}))
+
+
+def _line_u(n, u0=1e4, step=100.0):
+    return np.stack([u0 + step * np.arange(n), np.zeros(n), np.zeros(n)], 1)
+
+
+def test_mode2_time_blocks_from_accumulated_phase():
+    # step 100 along u, lmax 0.01: phase step 1 per row (the last row repeats it); dPhi 2.6:
+    # cumsum 1..7 / 2.6 = 0.38, 0.77, 1.15, 1.54, 1.92, 2.31, 2.69 -> blocks 0,0,1,1,1,2,2.  Spans
+    # 260 / (100 + i) lie in (2.4, 2.6]: ceil(8 / span) = 4 channel blocks of ceil(8 / 4) = 2
+    # channels in every time block.
+    got = _mode2(_line_u(7), dphi=2.6, lmax=0.01)
+    want = []
+    for rows in ([0, 1], [2, 3, 4], [5, 6]):
+        for c0 in (0, 2, 4, 6):
+            want.append((c0, c0 + 2, rows))
+    assert got == want
+
+
+def test_mode2_channel_size_and_band_cuts():
+    # one row block (small steps); span 290 / 100 = 2.9 for every row: ceil(8 / 2.9) = 3 blocks of
+    # ceil(8 / 3) = 3 channels -> cuts 0, 3, 6, 8, plus the band edge at 4
+    uvw = _line_u(3, step=1e-3)
+    got = _mode2(uvw, dphi=2.9, lmax=0.01, chan_map=[0, 0, 0, 0, 1, 1, 1, 1])
+    assert got == [(0, 3, [0, 1, 2]), (3, 4, [0, 1, 2]), (4, 6, [0, 1, 2]), (6, 8, [0, 1, 2])]
+
+
+def test_mode2_spans_below_one_channel_give_single_channels():
+    got = _mode2(_line_u(2, step=1e-3), dphi=0.5, lmax=0.01)
+    assert got == [(c, c + 1, [0, 1]) for c in range(8)]
+
+
+def test_mode2_w_steps_weighted_by_n_max():
+    # lmax 0.6 -> nmax = 1 - sqrt(1 - 0.36) = 0.2; steps of 5 in w only: phase step 1 per row;
+    # dPhi 1.6: cumsum 1, 2, 3, 4 / 1.6 = 0.625, 1.25, 1.875, 2.5 -> blocks 0, 1, 1, 2 (a step
+    # counted with lmax instead of nmax would give 3 per row and four blocks).  ext = |w| nmax with
+    # w = 1000 + 5 i -> span ~ 100 * 1.6 / 200 < 1: single channels
+    n = 4
+    uvw = np.stack([np.zeros(n), np.zeros(n), 1000.0 + 5.0 * np.arange(n)], 1)
+    got = _mode2(uvw, dphi=1.6, lmax=0.6, nchan=2)
+    assert got == [(0, 1, [0]), (1, 2, [0]), (0, 1, [1, 2]), (1, 2, [1, 2]), (0, 1, [3]), (1, 2, [3])]
+
+
+def test_mode2_single_row_baseline():
+    # no step to repeat: one time block; span 260 / 100 = 2.6 -> 4 blocks of 2 channels
+    got = _mode2(_line_u(1), dphi=2.6, lmax=0.01)
+    assert got == [(c, c + 2, [0]) for c in (0, 2, 4, 6)]
+
+
+def test_mode2_driver_decorr_fov_and_baseline_order():
+    """bda_map_mode2: dPhi = sqrt(6 decorr), lmax = cell npix sqrt(2)/2, autocorrelations skipped,
+    baselines in string-key order; the same blocks as the direct builder call."""
+    dphi, lmax = 2.6, 0.01
+    decorr = dphi ** 2 / 6
+    npix = 1000
+    cell = lmax / (npix * np.sqrt(2) / 2)
+    u1 = _line_u(7)
+    u2 = _line_u(1)
+    uvw = np.concatenate([u1, u2, _line_u(2)])
+    a0 = np.array([2] * 7 + [10] + [3, 3], np.int32)
+    a1 = np.array([10] * 7 + [11] + [3, 3], np.int32)
+    freqs = np.linspace(0.5 * NU1, NU1, 8)
+    m = bda.bda_map_mode2(uvw, a0, a1, freqs, NU1 / 100.0, decorr, cell, npix, np.zeros(8, np.int32))
+    got = _blocks(m)
+    # "10:11" (row 7, span 2.6) first, then "2:10" (rows 0-6) as in the 7-row case
+    want = [(c, c + 2, [7]) for c in (0, 2, 4, 6)]
+    for rows in ([0, 1], [2, 3, 4], [5, 6]):
+        for c0 in (0, 2, 4, 6):
+            want.append((c0, c0 + 2, rows))
+    assert got == want
