@@ -30,6 +30,7 @@ import time
 import numpy as np
 
 from . import bda as bdamod
+from . import jonesmaps
 from . import synth
 from .problems import Facet, Problem, chan_equidistant, make_cf_facet
 
@@ -128,10 +129,14 @@ def cfg2(nfx=9, image_npix=4096, ntime=600, nchan=64, nant=64, sort_by_baseline=
     log("cfg2: BDA map (%d blocks, %d words) in %.1fs" % (bdamod.nblocks_of(bmap), bmap.size, time.time() - t0))
     jones = None
     if with_jones:
+        # 10-min solution intervals and 8-channel solution domains, mapped as the Jones server's
+        # inputs are (ClassJones.GiveTimeMapping / GiveVisToJonesChanMapping)
         ntj = 8
         tsec = obs.tidx * 8.0
-        tmap = np.minimum((tsec // 600.0).astype(np.int32), ntj - 1)
-        cmap = (np.arange(nchan) // 8).astype(np.int32)
+        tmap = jonesmaps.time_mapping(600.0 * np.arange(ntj), 600.0 * np.arange(1, ntj + 1), tsec)
+        dfc = (freqs[-1] - freqs[0]) / (nchan - 1)
+        doms = np.stack([freqs[0::8] - dfc / 2, freqs[7::8] + dfc / 2], 1)
+        cmap = jonesmaps.vis_to_jones_chan(freqs, doms).astype(np.int32)
         jones = dict(mode=1, interp=0, apply_amp=1, apply_phase=1, scale=0, calib_error=0.0,
                      kms=synth.random_jones(ntj, nfx * nfx, nant, nchan // 8, 2004),
                      kms_time=tmap, kms_chan=cmap)
