@@ -52,7 +52,7 @@ __device__ __forceinline__ BlockRec read_block(const VisP& v, int64_t b) {
 }
 
 // Phasor exp(i * freq * angle) at three precisions:
-//   0 mixed: FP64 phase and range reduction, FP32 sincos (~1e-7 for any phase);
+//   0 mixed: FP64 phase and range reduction, FP32 sin/cos (phasor: ~1e-7 for any phase);
 //   1 FP64 throughout;
 //   2 FP32 throughout (phase formed and reduced in FP32: the precision-study variant).
 template <int PREC>
@@ -72,7 +72,7 @@ __device__ __forceinline__ cd phasor_prec(double freq, double angle) {
 
 // Phasors exp(i freq_c angle) along the channels of one row, for a lane that walks them in
 // increasing order.  PREC 0 with equidistant channels: FP64 phase base (freq_c = f0 + c df, from
-// VisP, so no frequency is loaded), FP32 sincos at an anchor (the first channel walked, a channel
+// VisP, so no frequency is loaded), FP32 sin/cos at an anchor (the first channel walked, a channel
 // after a gap, every multiple of 16), and in between the phasor advanced in FP32 by the per-row
 // step exp(i df angle).  Other precisions, or non-equidistant channels: every phasor formed
 // directly by phasor_prec.

@@ -178,7 +178,7 @@ typedef struct ddf_plan_opts {
   int32_t chunk;           /* max records per tile work item (0: 2048)                          */
   int32_t blocks_per_item; /* BDA blocks per gridding work item (0: 128)                        */
   int32_t group;           /* facets per lane group in gridding (0: auto, <= 64)                */
-  int32_t precision;       /* 0 mixed: FP64 uvw rotation and per-row phase base, FP32 sincos and
+  int32_t precision;       /* 0 mixed: FP64 uvw rotation and per-row phase base, FP32 sin/cos and
                               FP32 phasor advance along channels (re-anchored every 16), FP32
                               run sums / Jones products / Stokes maps, FP64 stamp, facet and
                               weight sums;
