@@ -677,52 +677,11 @@ __device__ __forceinline__ uint64_t lane_u64(uint64_t x, int j) {
   return uint64_t(lane_u32(uint32_t(x), j)) | (uint64_t(lane_u32(uint32_t(x >> 32), j)) << 32);
 }
 __device__ __forceinline__ float lane_f32(float x, int j) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j)); }
-__device__ __forceinline__ int uniform_i32(int x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ __forceinline__ int64_t uniform_i64(int64_t x) {
-  return int64_t(uint64_t(uint32_t(uniform_i32(int(x)))) | (uint64_t(uint32_t(uniform_i32(int(x >> 32)))) << 32));
-}
 
-// One DPP step of a 64-bit value: lanes of rows outside ROWMASK receive 0.
-template <int CTRL, int ROWMASK>
-__device__ __forceinline__ double dpp_f64(double x) {
-  const uint64_t b = uint64_t(__double_as_longlong(x));
-  const int lo = __builtin_amdgcn_update_dpp(0, int(uint32_t(b)), CTRL, ROWMASK, 0xF, true);
-  const int hi = __builtin_amdgcn_update_dpp(0, int(uint32_t(b >> 32)), CTRL, ROWMASK, 0xF, true);
-  return __longlong_as_double((long long)(uint64_t(uint32_t(lo)) | (uint64_t(uint32_t(hi)) << 32)));
-}
-
-// Sum over the 64 lanes of a wavefront (every lane active), returned to every lane.  Quad
-// swaps, row rotations by 4 and 8 (row sums), then row broadcasts 15 and 31 carry rows 0-2 into
-// row 3; lane 63 ends with the total.
-__device__ __forceinline__ double wave_total(double x) {
-  x += dpp_f64<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
-  x += dpp_f64<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
-  x += dpp_f64<0x124, 0xF>(x);  // row_ror:4
-  x += dpp_f64<0x128, 0xF>(x);  // row_ror:8
-  x += dpp_f64<0x142, 0xA>(x);  // row_bcast:15 -> rows 1, 3
-  x += dpp_f64<0x143, 0xC>(x);  // row_bcast:31 -> rows 2, 3
-  const uint64_t b = uint64_t(__double_as_longlong(x));
-  return __longlong_as_double((long long)(uint64_t(lane_u32(uint32_t(b), 63)) | (uint64_t(lane_u32(uint32_t(b >> 32), 63)) << 32)));
-}
-
-// FP32 counterpart of wave_total (same DPP sequence, one 32-bit move per step).
-template <int CTRL, int ROWMASK>
-__device__ __forceinline__ float dpp_f32(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROWMASK, 0xF, true));
-}
-__device__ __forceinline__ float wave_total_f32(float x) {
-  x += dpp_f32<0xB1, 0xF>(x);
-  x += dpp_f32<0x4E, 0xF>(x);
-  x += dpp_f32<0x124, 0xF>(x);
-  x += dpp_f32<0x128, 0xF>(x);
-  x += dpp_f32<0x142, 0xA>(x);
-  x += dpp_f32<0x143, 0xC>(x);
-  return lane_f32(x, 63);
-}
-
-// Stamp record of a binned (block, facet) record (plan_stamp_records): base | S << 16 | cfoff << 32,
-// with base the tile offset of the stamp's first cell (row (locy - sup) - y0, column
-// (locx - sup) - x0 of the (T + 2 h)^2 tile image) and cfoff the element offset of the record's
+// Stamp record of a binned (block, facet) record (plan_stamp_records): base | S << 16 | sub << 21 |
+// cfoff << 32, with base the tile offset of the stamp's first cell (row (locy - sup) - y0, column
+// (locx - sup) - x0 of the (T + 2 h)^2 tile image), sub the oversampled sub-kernel index
+// io * OverS + jo (used with the transposed CF planes) and cfoff the element offset of the record's
 // oversampled S x S sub-kernel in its facet's CF buffer.  Cell c of the stamp (row c / S, column
 // c % S) sits at tile offset base + (c / S) W + c % S and takes CF value cf[cfoff + c].
 struct Stamp {
