@@ -278,6 +278,58 @@ __global__ void reset_counter(uint32_t* c, int n) {
 // One workgroup handles one facet group; lane = slot * G + facet-in-group.
 // JMODE: 0 no Jones, 1 per-visibility Jones (applied once per run of one Jones key), 2 per-block
 // Jones, 3 per-visibility Jones with ScaleJones (applied per visibility).
+// Order in which grid pass A walks the visibilities of a block: the block's channel range is cut
+// where the DD or beam Jones channel changes (at most kWalkSeg segments, else one segment) and the
+// segments are walked one after the other, row-major within a segment.  The Jones key then stays
+// constant over the rows of a segment that share a Jones time (a row-major walk of a block across a
+// Jones-channel edge alternates between keys at every row), and the channel phasor still steps
+// along the channels of a segment's row.  Visibility g of the walk lies in the segment s with
+// nr (cs[s] - ch0) <= g < nr (cs[s + 1] - ch0).
+constexpr int kWalkSeg = 4;
+struct WalkOrder {
+  int nseg, nr, ch0;
+  int cs[kWalkSeg + 1];  // segment start channels; cs[nseg] .. cs[kWalkSeg] = ch1
+  __device__ __forceinline__ void at(int g, int& rr, int& cc) const {
+    int c0 = cs[0], w = cs[1] - cs[0];
+#pragma unroll
+    for (int k = 1; k < kWalkSeg; ++k)
+      if (k < nseg && g >= nr * (cs[k] - ch0)) {
+        c0 = cs[k];
+        w = cs[k + 1] - cs[k];
+      }
+    const int loc = g - nr * (c0 - ch0);
+    rr = loc / w;
+    cc = c0 + (loc - rr * w);
+  }
+};
+__device__ __forceinline__ WalkOrder walk_order(const JonesP& jp, const BlockRec& br, bool by_key) {
+  WalkOrder w;
+  w.nr = br.nr;
+  w.ch0 = br.ch0;
+  w.nseg = 1;
+#pragma unroll
+  for (int k = 0; k <= kWalkSeg; ++k) w.cs[k] = br.ch1;
+  w.cs[0] = br.ch0;
+  if (!by_key) return w;
+  for (int c = br.ch0 + 1; c < br.ch1; ++c) {
+    const bool cut = (jp.kms && gptr(jp.kchan)[c] != gptr(jp.kchan)[c - 1]) ||
+                     (jp.beam && gptr(jp.bchan)[c] != gptr(jp.bchan)[c - 1]);
+    if (!cut) continue;
+    if (w.nseg == kWalkSeg) {
+      // more edges than segments: the block is walked row-major
+      w.nseg = 1;
+#pragma unroll
+      for (int k = 1; k <= kWalkSeg; ++k) w.cs[k] = br.ch1;
+      break;
+    }
+#pragma unroll
+    for (int k = 1; k < kWalkSeg; ++k)
+      if (k == w.nseg) w.cs[k] = c;
+    ++w.nseg;
+  }
+  return w;
+}
+
 template <int READ, int JMODE, int PREC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void grid_pass_a(BatchParams p, const VisP v, const uint64_t* place, float2* V,
                                                    uint32_t* queue) {
@@ -289,8 +341,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
   // Slots are aligned to wavefronts (64 / G slots of G lanes per wavefront): the lanes of a slot
   // are the facets of the group for one block; they stage the block's visibilities (values,
   // weight, flag) in LDS, G at a time, one per lane, and then every lane walks the staged
-  // visibilities in the reference order.  A wavefront's LDS operations complete in order,
-  // so the staging needs no workgroup barrier.
+  // visibilities in the order of WalkOrder (the reference order, row-major, unless a Jones-channel
+  // edge cuts the block).  A wavefront's LDS operations complete in order, so the staging needs no
+  // workgroup barrier.
   extern __shared__ __attribute__((aligned(16))) double lds[];
   const int G = p.group;
   const int spw = 64 / G;
@@ -302,10 +355,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
   // lanes (facets) of a wavefront fall on different LDS banks
   const int cs = 2 * v.nchan + 1;
   const int nchan_acc = p.chan_lds ? G * cs : 0;
-  // staged entry (floats): READ complex values, weight, flag, row, pad, uvw (3 doubles) and for
-  // JMODE 1 the Jones key (DD time, DD chan, beam time, beam chan, antenna 0, antenna 1)
-  // (weight, flag, row, pad) first, so that the walk reads an entry with a few 16-byte LDS reads
-  constexpr int kW = 0, kF = 1, kRow = 2, kV = 4;
+  // staged entry (floats): weight, flag word, row, channel, READ complex values, the Jones key
+  // (JMODE 1: DD time, DD chan, beam time, beam chan, antenna 0, antenna 1) and uvw (3 doubles);
+  // (weight, flag word, row, channel) first, so that the walk reads an entry with a few 16-byte LDS
+  // reads.  Flag word: bit 0 the visibility is skipped (flagged, or a row outside the data), bit 1
+  // (JMODE 1) its Jones key differs from that of the visibility before it in the walk or that one
+  // is skipped, i.e. it opens a run of one key.
+  constexpr int kW = 0, kF = 1, kRow = 2, kCh = 3, kV = 4;
   constexpr int kKey = grid_a_key_offset(READ), kU = kKey + 6, kStride = grid_a_stride(READ);
   // 16-byte aligned staging buffers
   float* s_stage = reinterpret_cast<float*>(lds + ((p.nf * per_f + nchan_acc + 1) & ~1));
@@ -371,13 +427,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
       BlockRec br = read_block(v, b);
       const int64_t mid = br.rows[br.nr / 2];
       const bool mid_ok = mid >= 0 && mid < v.nrow;
+      JonesP jp = p.jp;
+      jp.kdir = fd.kms_dir;
+      jp.bdir = fd.beam_dir;
+      const WalkOrder wo = walk_order(jp, br, JMODE == 1);
       DecorrP dp = p.dp;
       dp.l0 = fd.dec_l0;
       dp.m0 = fd.dec_m0;
       const double dec = (doit && mid_ok) ? decorr_factor(dp, v.uvw, p.fmean0, mid) : 1.;
-      JonesP jp = p.jp;
-      jp.kdir = fd.kms_dir;
-      jp.bdir = fd.beam_dir;
       double swt = 0, ssw = 0, sj = 0;
       cd acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
       const Unpacked u = unpack_place(rec);
@@ -413,7 +470,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
       double rfw = 0;  // sum of the run's weights
       double sjr = 0;  // JMODE 1: sum of BB w over the block's runs
       const double dec2 = dec * dec, ndec2 = double(npg) * dec2;
-      int kt = -1, kc = -1, bt = -1, bc = -1, ka0 = -1, ka1 = -1;
       M2T<RT> cJ0, cJ1;
       double cBB = 0;
       bool any = false;
@@ -452,16 +508,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
       const int nvis = br.nr * nch;
       for (int i0 = 0; i0 < nvis; i0 += K) {
         const int n = min(K, nvis - i0);
-        // stage visibilities i0 .. i0 + n - 1 of the block (row-major): values, weight, flag,
-        // row, uvw and (JMODE 1) the Jones key, so that the walk below reads LDS only
+        // stage visibilities i0 .. i0 + n - 1 of the walk (values, weight, flag word, row, channel,
+        // uvw and the Jones key), so that the walk below reads LDS only
+        auto key_of = [&](int64_t r, int c, int* k) {
+          k[0] = jp.kms ? gptr(jp.ktime)[r] : -1;
+          k[1] = jp.kms ? gptr(jp.kchan)[c] : -1;
+          k[2] = jp.beam ? gptr(jp.btime)[r] : -1;
+          k[3] = jp.beam ? gptr(jp.bchan)[c] : -1;
+          k[4] = gptr(jp.a0)[r];
+          k[5] = gptr(jp.a1)[r];
+        };
         for (int i = lf; i < n; i += G) {
           const int g = i0 + i;
-          const int rr = g / nch;
-          const int cc = br.ch0 + (g - rr * nch);
+          int rr, cc;
+          wo.at(g, rr, cc);
           const int64_t srow = br.rows[rr];
           float* e = stg + i * kStride;
           if (srow < 0 || srow >= v.nrow) {
-            e[kF] = 1.f;
+            e[kF] = __int_as_float(1);
             continue;
           }
           const size_t vc = size_t(srow) * v.nchan + cc;
@@ -473,26 +537,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
             e[kV + 2 * k + 1] = vp[k].y;
           }
           e[kW] = gptr(v.weights)[vc];
-          e[kF] = gptr(v.flags)[vc * v.ncorr] ? 1.f : 0.f;
+          int fl = gptr(v.flags)[vc * v.ncorr] ? 1 : 0;
           e[kRow] = __int_as_float(int(srow));
+          e[kCh] = __int_as_float(cc);
           double* eu = reinterpret_cast<double*>(e + kU);
           eu[0] = q[0];
           eu[1] = q[1];
           eu[2] = q[2];
           if (JMODE == 1) {
             int* ek = reinterpret_cast<int*>(e + kKey);
-            ek[0] = jp.kms ? gptr(jp.ktime)[srow] : -1;
-            ek[1] = jp.kms ? gptr(jp.kchan)[cc] : -1;
-            ek[2] = jp.beam ? gptr(jp.btime)[srow] : -1;
-            ek[3] = jp.beam ? gptr(jp.bchan)[cc] : -1;
-            ek[4] = gptr(jp.a0)[srow];
-            ek[5] = gptr(jp.a1)[srow];
+            int k[6];
+            key_of(srow, cc, k);
+#pragma unroll
+            for (int j = 0; j < 6; ++j) ek[j] = k[j];
+            bool opens = true;
+            if (g > 0) {
+              int pr, pc;
+              wo.at(g - 1, pr, pc);
+              const int64_t prow = br.rows[pr];
+              if (prow >= 0 && prow < v.nrow && !gptr(v.flags)[(size_t(prow) * v.nchan + pc) * v.ncorr]) {
+                int kp[6];
+                key_of(prow, pc, kp);
+                opens = false;
+#pragma unroll
+                for (int j = 0; j < 6; ++j) opens = opens || kp[j] != k[j];
+              }
+            }
+            if (opens) fl |= 2;
           }
+          e[kF] = __int_as_float(fl);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         if (doit) {
-          int c = br.ch0 + (i0 - (i0 / nch) * nch);
           for (int i = 0; i < n; ++i) {
             const float* e = stg + i * kStride;
             // the whole entry in one batch of LDS reads (one wait)
@@ -510,15 +587,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
               const float2 a = *reinterpret_cast<const float2*>(e + kV);
               ev[0] = a.x; ev[1] = a.y;
             }
-            int4 ka = make_int4(0, 0, 0, 0);
-            int2 kb = make_int2(0, 0);
-            if (JMODE == 1) {
-              ka = *reinterpret_cast<const int4*>(e + kKey);
-              kb = *reinterpret_cast<const int2*>(e + kKey + 4);
-            }
-            const int cv_ = c;
-            if (++c == br.ch1) c = br.ch0;
-            if (hd.y != 0.f) continue;
+            const int fl = __float_as_int(hd.y);
+            if (fl & 1) continue;
+            const int cv_ = __float_as_int(hd.w);
             const int erow = __float_as_int(hd.z);
             if (erow != row) {
               row = erow;
@@ -531,14 +602,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
             const double fw = double(hd.x);
             const double fwd = fw * dec2;
             if (JMODE == 1) {
-              const int k_t = ka.x, k_c = ka.y, b_t = ka.z, b_c = ka.w, q0 = kb.x, q1 = kb.y;
-              if (!any || k_t != kt || k_c != kc || b_t != bt || b_c != bc || q0 != ka0 || q1 != ka1) {
+              if (fl & 2) {
+                // a new run: the previous run's sum goes through its Jones pair first
+                const int4 ka = *reinterpret_cast<const int4*>(e + kKey);
+                const int2 kb = *reinterpret_cast<const int2*>(e + kKey + 4);
                 if (any) flush();
-                const JonesPair jj = jones_keyed(jp, q0, q1, k_t, k_c, b_t, b_c);
+                const JonesPair jj = jones_keyed(jp, kb.x, kb.y, ka.x, ka.y, ka.z, ka.w);
                 cJ0 = cvt<RT>(jj.J0);
                 cJ1 = cvt<RT>(jj.J1);
                 cBB = jj.BB;
-                kt = k_t; kc = k_c; bt = b_t; bc = b_c; ka0 = q0; ka1 = q1;
                 any = true;
               }
               rfw += fw;

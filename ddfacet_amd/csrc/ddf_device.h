@@ -326,7 +326,12 @@ __device__ __forceinline__ JonesPair jones_keyed(const JonesP& jp, int a0, int a
     r.J0 = prepared_jones(jp.pb, jp.bdims, bt, a0, jp.bdir, bc);
     r.J1 = prepared_jones(jp.pb, jp.bdims, bt, a1, jp.bdir, bc);
   }
-  const double bb = (cabs(r.J0.a[0]) * cabs(r.J1.a[0]) + cabs(r.J0.a[3]) * cabs(r.J1.a[3])) / 2.;
+  // |x| |y| as sqrt(|x|^2 |y|^2): two square roots in place of four hypot calls, within an ulp or
+  // two of the product of the moduli.  The product of the squared moduli leaves the normal FP64
+  // range only where |x| |y| is above 1e154 or below 1e-154, where BB = bb^2 itself (nearly)
+  // overflows or underflows.
+  auto n2 = [](cd z) { return z.re * z.re + z.im * z.im; };
+  const double bb = (sqrt(n2(r.J0.a[0]) * n2(r.J1.a[0])) + sqrt(n2(r.J0.a[3]) * n2(r.J1.a[3]))) / 2.;
   r.BB = bb * bb;
   return r;
 }

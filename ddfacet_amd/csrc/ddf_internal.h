@@ -74,20 +74,29 @@ __device__ __forceinline__ cd phasor_prec(double freq, double angle) {
 // increasing order.  PREC 0 with equidistant channels: FP64 phase base (freq_c = f0 + c df, from
 // VisP, so no frequency is loaded), FP32 sin/cos at an anchor (the first channel walked, a channel
 // after a gap, every multiple of 16), and in between the phasor advanced in FP32 by the per-row
-// step exp(i df angle).  Other precisions, or non-equidistant channels: every phasor formed
-// directly by phasor_prec.
+// step exp(i df angle), formed at the first step of the row (rows of a single channel never
+// form it).  Other precisions, or non-equidistant channels: every phasor formed directly by
+// phasor_prec.
 template <int PREC>
 struct ChanPhasor {
   double angle;
   int last;
+  bool have_step;
   float cr, ci, sr, si;
   __device__ __forceinline__ void row(double a) {
     angle = a;
     last = -2;
+    have_step = false;
   }
   __device__ __forceinline__ cd at(const VisP& v, int c) {
     if (PREC != 0 || !v.equi) return phasor_prec<PREC>(gptr(v.freqs)[c], angle);
     if (c == last + 1 && (c & 15) != 0) {
+      if (!have_step) {
+        const cd st = phasor(v.df * angle);
+        sr = float(st.re);
+        si = float(st.im);
+        have_step = true;
+      }
       const float r = cr * sr - ci * si, i = cr * si + ci * sr;
       cr = r;
       ci = i;
@@ -95,11 +104,6 @@ struct ChanPhasor {
       const cd a = phasor(fma(double(c), v.df, v.f0) * angle);
       cr = float(a.re);
       ci = float(a.im);
-      if (last == -2) {
-        const cd st = phasor(v.df * angle);
-        sr = float(st.re);
-        si = float(st.im);
-      }
     }
     last = c;
     return cd{cr, ci};
@@ -175,7 +179,7 @@ struct BatchParams {
   int64_t nblk;
 };
 
-// Staged visibility entry of gridding pass A (floats): weight, flag, row, pad; READ complex
+// Staged visibility entry of gridding pass A (floats): weight, flag word, row, channel; READ complex
 // values from offset 4; the Jones key (6 ints) from grid_a_key_offset; uvw (3 doubles) after it;
 // the stride a multiple of 4 floats so that every entry is 16-byte aligned.
 constexpr int grid_a_key_offset(int read) { return (4 + 2 * read + 3) & ~3; }
