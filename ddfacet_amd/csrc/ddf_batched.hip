@@ -1089,8 +1089,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
   using RT = typename std::conditional<PREC == 1, double, float>::type;
   constexpr int FB = 4;
   __shared__ cx<RT> sY[FB][256][4];
-  __shared__ uint8_t sNZ[FB][256];
+  // per run, one byte per facet of the batch (read as one word): the run's model values are not
+  // all zero
+  __shared__ __attribute__((aligned(4))) uint8_t sNZ[256][FB];
   __shared__ RunInfo sRun[256];
+  // l0, m0, n0 of the batch's facets: the visibility loop reads them from LDS, not from the facet
+  // table in global memory (whose loads the compiler cannot hoist or move to the scalar unit)
+  __shared__ double sGeo[FB][3];
+  // mixed precision: each lane's pre-scaled uvw, read back after every batch's barrier so that it
+  // is not held in registers (or spilled) across the forming of Y
+  __shared__ double sU[3][256];
   const int t = threadIdx.x;
   for (int64_t k0 = int64_t(blockIdx.x) * 256; k0 < nvl; k0 += int64_t(gridDim.x) * 256) {
     const int64_t k = k0 + t;
@@ -1150,10 +1158,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
       u2 = q[2];
       fr = gptr(v.freqs)[c];
     }
+    if (PREC == 0) {
+      // mixed precision: uvw pre-scaled by -kPhaseScale freq, so that the phase of facet f is
+      // u0 l0 + u1 m0 + u2 n0 (three FP64 operations per facet; the rounding differs from forming
+      // the angle first by a few ulp of the phase)
+      const double sc = -kPhaseScale * fr;
+      sU[0][t] = u0 * sc;
+      sU[1][t] = u1 * sc;
+      sU[2][t] = u2 * sc;
+    }
     cd acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     __syncthreads();
     for (int f0 = 0; f0 < p.nf; f0 += FB) {
       const int nfb = min(FB, p.nf - f0);
+      if (t < nfb) {
+        const FacetDev fd = gload(p.facets + f0 + t);
+        sGeo[t][0] = fd.l0;
+        sGeo[t][1] = fd.m0;
+        sGeo[t][2] = fd.n0;
+      }
       // Y of every (run, facet of the batch)
       for (int j = t; j < nr * nfb; j += 256) {
         const int fi = j / nr;
@@ -1211,15 +1234,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) sY[fi][ir][q] = cv.a[q];
-        sNZ[fi][ir] = uint8_t(nz);
+        sNZ[ir][fi] = uint8_t(nz);
       }
       __syncthreads();
       if (live) {
-        for (int fi = 0; fi < nfb; ++fi) {
-          if (!sNZ[fi][myrun]) continue;
-          const FacetDev fd = gload(p.facets + f0 + fi);
-          const double angle = -kPhaseScale * (u0 * fd.l0 + u1 * fd.m0 + u2 * fd.n0);
-          const cx<RT> corr = cvt<RT>(phasor_prec<PREC>(fr, angle));
+        const uint32_t nzw = *reinterpret_cast<const uint32_t*>(sNZ[myrun]);
+        if (PREC == 0) {
+          u0 = sU[0][t];
+          u1 = sU[1][t];
+          u2 = sU[2][t];
+        }
+#pragma unroll
+        for (int fi = 0; fi < FB; ++fi) {
+          if (fi >= nfb || !((nzw >> (8 * fi)) & 0xFFu)) continue;
+          const double dot = u0 * sGeo[fi][0] + u1 * sGeo[fi][1] + u2 * sGeo[fi][2];
+          const cx<RT> corr = cvt<RT>(PREC == 0 ? phasor(dot) : phasor_prec<PREC>(fr, -kPhaseScale * dot));
 #pragma unroll
           for (int q = 0; q < 4; ++q) acc[q] = acc[q] + cvt<double>(sY[fi][myrun][q] * corr);
         }

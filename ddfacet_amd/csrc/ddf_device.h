@@ -123,19 +123,24 @@ __device__ __forceinline__ int wave_sum(int v) {
 // exp(i*phase): the phase, formed in FP64, is reduced in FP64 to r = phase - q pi/2 with
 // |r| <= pi/4 (pi/2 split in a head and a tail), then sin r and cos r are evaluated in FP32 by
 // their Taylor polynomials (degrees 9 and 8: truncation below 3e-8 on |r| <= pi/4) and placed by
-// the quadrant q mod 4.  The error is ~1e-7 for any |phase|, where an FP32 phase of 1e4 rad would
-// already carry ~1e-3 rad.
+// the quadrant q mod 4.  The error is ~1e-7 for |phase| up to 1e15 rad (tested to 1e8; beyond,
+// the FP64 phase itself is coarser than a radian), where an FP32 phase of 1e4 rad would already
+// carry ~1e-3 rad.
+// q is rounded by adding and subtracting 1.5 2^52 (exact for |phase 2/pi| < 2^51: the sum then
+// lies in [2^52, 2^53), where the FP64 spacing is 1), whose low mantissa bits hold q mod 2^32.
 __host__ __device__ __forceinline__ cd phasor(double phase) {
   const double two_over_pi = 0.63661977236758134308;
   const double pio2_hi = 1.57079632679489655800;
   const double pio2_lo = 6.12323399573676603587e-17;
-  const double q = rint(phase * two_over_pi);
+  const double kRound = 6755399441055744.0;  // 1.5 2^52
+  const double t = phase * two_over_pi + kRound;
+  const double q = t - kRound;
   double r = fma(-q, pio2_hi, phase);
   r = fma(-q, pio2_lo, r);
   const float x = float(r), x2 = x * x;
   const float sp = x * fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, 1.f / 362880.f, -1.f / 5040.f), 1.f / 120.f), -1.f / 6.f), 1.f);
   const float cp = fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, 1.f / 40320.f, -1.f / 720.f), 1.f / 24.f), -0.5f), 1.f);
-  const int k = int(static_cast<long long>(q) & 3);
+  const int k = int(static_cast<uint32_t>(__builtin_bit_cast(unsigned long long, t)) & 3u);
   const float s = (k & 1) ? cp : sp, c = (k & 1) ? sp : cp;
   const float ss = (k & 2) ? -s : s;
   const float cc = ((k + 1) & 2) ? -c : c;

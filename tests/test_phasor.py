@@ -1,5 +1,5 @@
 """The device phasor (FP64 reduction, FP32 Taylor polynomials) checked on the host against
-long-double sin / cos: maximum error below 3e-7 for phases up to 1e5 rad (tests/phasor_check.cpp,
+long-double sin / cos: maximum error below 3e-7 for phases up to 1e8 rad (tests/phasor_check.cpp,
 compiled from the same header as the kernels)."""
 import os
 import shutil
