@@ -436,7 +436,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
       dp.m0 = fd.dec_m0;
       const double dec = (doit && mid_ok) ? decorr_factor(dp, v.uvw, p.fmean0, mid) : 1.;
       double swt = 0, ssw = 0, sj = 0;
-      cd acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+      // block sums: with per-visibility Jones (JMODE 1) a sum over the block's runs (about one per
+      // block), in the arithmetic type of the run sums; otherwise over visibilities, in FP64
+      using AT = typename std::conditional<JMODE == 1 && PREC != 1, float, double>::type;
+      cx<AT> acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
       const Unpacked u = unpack_place(rec);
       double BBblk = 0;
       JonesPair jblk;
@@ -484,12 +487,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
         }
         if (READ == 4) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) acc[q] = acc[q] + cvt<double>(m.a[q]);
+          for (int q = 0; q < 4; ++q) acc[q] = acc[q] + cvt<AT>(m.a[q]);
         } else if (READ == 2) {
-          acc[0] = acc[0] + cvt<double>(m.a[0]);
-          acc[3] = acc[3] + cvt<double>(m.a[3]);
+          acc[0] = acc[0] + cvt<AT>(m.a[0]);
+          acc[3] = acc[3] + cvt<AT>(m.a[3]);
         } else {
-          acc[0] = acc[0] + cvt<double>(m.a[0]);
+          acc[0] = acc[0] + cvt<AT>(m.a[0]);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) racc[q] = cx<RT>{0, 0};
@@ -679,12 +682,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
               }
               if (READ == 4) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc[q] = acc[q] + meas[q] * wt;
+                for (int q = 0; q < 4; ++q) acc[q] = acc[q] + cvt<AT>(meas[q] * wt);
               } else if (READ == 2) {
-                acc[0] = acc[0] + meas[0] * wt;
-                acc[3] = acc[3] + meas[3] * wt;
+                acc[0] = acc[0] + cvt<AT>(meas[0] * wt);
+                acc[3] = acc[3] + cvt<AT>(meas[3] * wt);
               } else {
-                acc[0] = acc[0] + meas[0] * wt;
+                acc[0] = acc[0] + cvt<AT>(meas[0] * wt);
               }
             }
             if (JMODE != 1) {
@@ -703,11 +706,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
           sj = dec2 * sjr;
         }
         if (JMODE == 2) {
-          M2 m{{acc[0], acc[1], acc[2], acc[3]}};
+          M2 m{{cvt<double>(acc[0]), cvt<double>(acc[1]), cvt<double>(acc[2]), cvt<double>(acc[3])}};
           if (p.dopsf) m = mul(mul(jblk.J0, m), herm(jblk.J1));
           m = mul(mul(herm(jblk.J0), m), jblk.J1);
   #pragma unroll
-          for (int q = 0; q < 4; ++q) acc[q] = m.a[q];
+          for (int q = 0; q < 4; ++q) acc[q] = cvt<AT>(m.a[q]);
           sj = ssw * BBblk;
         }
         // facet-major [f][block]: the tile pass reads one facet's records in block order
@@ -715,7 +718,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
         for (int k = 0; k < p.nstokes; ++k) {
           cd s = cd{0, 0};
   #pragma unroll
-          for (int q = 0; q < 4; ++q) s = s + cd{p.sm[k][q][0], p.sm[k][q][1]} * acc[q];
+          for (int q = 0; q < 4; ++q) s = s + cd{p.sm[k][q][0], p.sm[k][q][1]} * cvt<double>(acc[q]);
           out[k] = make_float2(float(s.re), float(s.im));
         }
         double* a = s_acc + f * per_f;
