@@ -399,8 +399,8 @@ def main():
                           "kernel": "elementwise mul by 1 (3 launches after the fill of x)"}))
         return
     if a.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path (see DESIGN.md for how "
-              "the path would shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU: the library has no multi-GPU path (the work would shard by "
+              "visibility rows, with one all-reduce of the facet grids); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     if a.workload == "uneven121":

@@ -1,8 +1,8 @@
 """HIP plain kernels (one facet per launch) through the drop-in shim, against the golden vectors
 of the reference gridder and against the CPU oracle.
 
-Tolerances: relative L2 <= 1e-5 on grids and on degridded model visibilities (the north-star
-parity bar); sumwt / SumJones are FP64 sums compared at 1e-10.  The set of grid cells a case
+Tolerances: relative L2 <= 1e-5 on grids and on degridded model visibilities (the parity bar
+for floating point); sumwt / SumJones are FP64 sums compared at 1e-10.  The set of grid cells a case
 touches must match the reference exactly: it is fixed by the integer placement maps.
 """
 import os

@@ -82,8 +82,8 @@ def test_precision_study_vs_fp64_dft():
             "dirty_rel_l2_vs_fp64": rel_l2(img, res["fp64"][1]),
         }
     print("precision study: " + json.dumps(table))
-    out = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
-    if os.path.isdir(out):
+    out = os.environ.get("DDF_RESULTS_DIR", "")
+    if out and os.path.isdir(out):
         with open(os.path.join(out, "precision_cfg3.json"), "w") as fh:
             json.dump(table, fh, indent=1)
     # the gridding approximation (S = 7 w-kernels up to the longest baselines, sources at the facet
@@ -108,8 +108,8 @@ def test_precision_variants_on_channel_blocks_vs_fp64():
         table[v] = {"predict_rel_l2_vs_fp64": rel_l2(pred, res["fp64"][0]),
                     "dirty_rel_l2_vs_fp64": rel_l2(img, res["fp64"][1])}
     print("precision on 64-channel blocks: " + json.dumps(table))
-    out = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
-    if os.path.isdir(out):
+    out = os.environ.get("DDF_RESULTS_DIR", "")
+    if out and os.path.isdir(out):
         with open(os.path.join(out, "precision_cfg3_blocks64.json"), "w") as fh:
             json.dump(table, fh, indent=1)
     assert table["mixed"]["predict_rel_l2_vs_fp64"] < 1e-5

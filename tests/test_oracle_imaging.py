@@ -101,7 +101,7 @@ def test_conj_plane_identity():
     np.testing.assert_allclose(lhs, rhs, atol=1e-14)
 
 
-def test_estimate_npix_matches_survey_sizes():
+def test_estimate_npix_matches_workload_sizes():
     from ddfacet_amd.configs import estimate_npix
     assert estimate_npix(4096 / 9, 1.7) == (455, 735)
     assert estimate_npix(6144 / 11, 1.7)[1] == 975
