@@ -360,6 +360,66 @@ def run_autograd(a, gd_steps=20):
     print(json.dumps(out))
 
 
+def run_weights(a):
+    """Briggs imaging weights (robust 0) of the 81-facet workload's chunk on the device: uv-grid
+    index, scatter-add, robust transform, renormalisation, inputs resident in HBM.  CPU baseline:
+    the numpy oracle of the same steps on the whole chunk."""
+    import torch
+    from ddfacet_amd import configs
+    from ddfacet_amd import weights as dw
+    P = configs.cfg2(with_vis=False, with_jones=False, log=log)
+    R, Cn = P.uvw.shape[0], P.freqs.shape[0]
+    w = np.where(P.flags[..., 0], np.float32(0), P.weights).astype(np.float32)
+    # weight grid of a 4096-pixel image at 1.5" (Super 1): cell = 1 / FOV in wavelengths
+    fov = P.extra["cell_arcsec"] * np.pi / 180 / 3600 * P.extra["image_npix"]
+    cell = 1.0 / fov
+    uvmax = dw.uvmax_wavelengths(P.uvw, P.freqs)
+    xymax, npixx, npix = dw.grid_geometry(uvmax, cell)
+    cpu = None
+    if not a.no_cpu:
+        from oracle import weights as ow
+        sl = slice(0, R)
+        t0 = time.perf_counter()
+        ow.imaging_weights(P.uvw[sl], w[sl], P.freqs, cell, xymax, npixx, npix, weighting="briggs", robust=0.0)
+        dt = time.perf_counter() - t0
+        nsamp = w[sl].size
+        cpu = {"value": nsamp / dt, "unit": "vis/s", "cores": 1, "kind": "port",
+               "sample": "numpy restatement (oracle/weights.py) on the whole chunk (%d weights), %.1f s" % (nsamp, dt)}
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    tu = torch.from_numpy(np.ascontiguousarray(P.uvw)).to(dev)
+    tw = torch.from_numpy(w).to(dev)
+    run = lambda: dw.imaging_weights(tu, tw, P.freqs, cell, uvmax, weighting="briggs", robust=0.0)
+    for _ in range(a.warmup):
+        run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        out = run()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    # algorithmic bytes: index (uvw per row, weight in, index out), widening to int64, scatter-add
+    # (weight, index, FP64 read-modify-write of the cell), robust transform (three passes over the
+    # grid), renormalisation (weight, index, cell gather, weight out), the weight copy
+    nvis = R * Cn
+    nbytes = 24 * R + nvis * (4 + 4 + 4 + 8 + 4 + 8 + 16 + 4 + 4 + 8 + 4 + 8) + 3 * 8 * npix
+    ach = nbytes / (ms * 1e-3) / 1e9
+    res = {
+        "metric": "vis/sec Briggs imaging weights, 81-facet MeerKAT-like chunk",
+        "value": nvis / (ms * 1e-3), "unit": "vis/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64 grid, f32 weights",
+        "data": "synthetic",
+        "config": {"workload": "imaging weights of the 81-facet MeerKAT-like chunk (64 antennas, 600 x 8 s, 64 ch), "
+                               "Briggs robust 0, weight grid %d x %d" % (npixx, xymax + 1),
+                   "rows": R, "channels": Cn, "nvis": nvis, "npix": npix},
+        "roofline": {"bound": "hbm", "kernel": "whole pipeline", "achieved": ach, "peak": 8000.0, "unit": "GB/s",
+                     "frac": ach / 8000.0, "traffic": None, "algorithmic_bytes_per_launch": nbytes},
+        "cpu_baseline": cpu,
+        "finite_weights": int(torch.isfinite(out).sum()),
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -368,12 +428,13 @@ def main():
     ap.add_argument("--mode", default="batched", choices=["batched", "plain"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--facets", type=int, default=0, help="use only the first N facets (debug)")
-    ap.add_argument("--workload", default="meerkat81", choices=["meerkat81", "uneven121", "autograd25"],
+    ap.add_argument("--workload", default="meerkat81", choices=["meerkat81", "uneven121", "autograd25", "weights"],
                     help="meerkat81: the 81-facet MeerKAT-like cycle (the headline line); uneven121: "
                          "121 facets x 4 bands with baseline-dependent averaging, plain per-facet "
                          "launches against the autotuned batched cycle (eager and graph-replayed); "
                          "autograd25: 20 chi^2 gradient-descent steps through the differentiable "
-                         "degridder on 25 facets x 4 bands")
+                         "degridder on 25 facets x 4 bands; weights: Briggs imaging weights of the "
+                         "81-facet workload's chunk")
     ap.add_argument("--tile", type=int, default=0, help="uv tile edge of the batched plan (0: the plan default)")
     ap.add_argument("--blocks-per-item", type=int, default=0,
                     help="BDA blocks per pass-A work item of the batched plan (0: the plan default)")
@@ -407,6 +468,8 @@ def main():
         return run_uneven(a)
     if a.workload == "autograd25":
         return run_autograd(a)
+    if a.workload == "weights":
+        return run_weights(a)
     from ddfacet_amd import configs
     t0 = time.time()
     # geometry, maps, CFs and Jones on the host; the CPU baseline runs before the GPU is touched

@@ -901,6 +901,129 @@ extern "C" int ddf_accumulate_weights(double* grid, int64_t grid_len, const floa
   return DDF_OK;
 }
 
+// ---- imaging weights: uv-grid index, Briggs transform, renormalisation ---------------------------
+namespace {
+constexpr double kLightSpeedWeights = 299792458.0;  // ClassVisServer._cc
+
+// One thread per (row, channel): the uv-grid cell of ClassVisServer._uv_to_index
+// (ClassVisServer.py:983-1004), in its operation order and without contraction: (u freq) / c, then
+// / cell, floored; the int64 cell number is narrowed to int32 as numpy's assignment into an int32
+// array does (two's complement), and the band offset added and narrowed again.
+__global__ __launch_bounds__(256) void weights_index_k(const double* uvw, const float* w, int64_t nrow, int nchan,
+                                                       const double* freqs, const int32_t* chan_band, double cell,
+                                                       int64_t npix, int64_t npixx, int nbands, int64_t xymax,
+                                                       int32_t* index) {
+#pragma clang fp contract(off)
+  const int64_t n = nrow * int64_t(nchan);
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r = i / nchan;
+    const int c = int(i - r * nchan);
+    double u = uvw[3 * r], v = uvw[3 * r + 1];
+    if (v < 0) {
+      u = -u;
+      v = -v;
+    }
+    const double f = freqs[c];
+    const double ul = (u * f) / kLightSpeedWeights, vl = (v * f) / kLightSpeedWeights;
+    const int64_t x = int64_t(floor(ul / cell)) + xymax;
+    const int64_t y = int64_t(floor(vl / cell));
+    int32_t k = int32_t(uint32_t(uint64_t(y * npixx + x)));
+    if (nbands > 1) k = int32_t(uint32_t(uint64_t(int64_t(k) + int64_t(chan_band[c]) * npix)));
+    if (w[i] == 0.f) k = 0;
+    index[i] = k;
+  }
+}
+
+// One workgroup per band: avgW = sum g^2 / sum g over the band's cells, then g <- 1 + g sSq with
+// sSq = num2 / avgW (ClassVisServer.py:1188-1197), the product rounded before the sum.
+__global__ __launch_bounds__(1024) void weights_briggs_k(double* grid, int64_t npix, double num2) {
+#pragma clang fp contract(off)
+  __shared__ double s2[1024], s1[1024];
+  double* g = grid + int64_t(blockIdx.x) * npix;
+  double a2 = 0, a1 = 0;
+  for (int64_t i = threadIdx.x; i < npix; i += blockDim.x) {
+    const double x = g[i];
+    a2 += x * x;
+    a1 += x;
+  }
+  s2[threadIdx.x] = a2;
+  s1[threadIdx.x] = a1;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (int(threadIdx.x) < o) {
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  const double avg = s2[0] / s1[0];
+  const double ssq = num2 / avg;
+  for (int64_t i = threadIdx.x; i < npix; i += blockDim.x) {
+    const double t = g[i] * ssq;
+    g[i] = 1. + t;
+  }
+}
+
+// weights[i] /= grid[index[i]] (ClassVisServer.py:1035-1040): indices >= grid_len read cell 0 as
+// the reference sets them; negative ones count from the end as numpy indexing does (below
+// -grid_len, where numpy raises, cell 0).  The quotient is formed in FP64 and rounded to FP32, as
+// numpy's in-place division of a float32 array by float64 values is.
+__global__ __launch_bounds__(256) void weights_finalize_k(float* w, const int32_t* idx, int64_t n, const double* grid,
+                                                          int64_t glen) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    int64_t k = idx[i];
+    if (k >= glen) k = 0;
+    if (k < 0) k += glen;
+    if (k < 0) k = 0;
+    w[i] = float(double(w[i]) / grid[k]);
+  }
+}
+
+unsigned weight_blocks(int64_t n) {
+  int64_t nb = (n + 255) / 256;
+  return unsigned(nb > 65536 ? 65536 : (nb < 1 ? 1 : nb));
+}
+}  // namespace
+
+extern "C" int ddf_weights_index(const double* uvw, const float* weights, int64_t nrow, int nchan, const double* freqs,
+                                 const int32_t* chan_band, double cell, int64_t npix, int64_t npixx, int nbands,
+                                 int64_t xymax, int32_t* index, void* stream) {
+  if (nrow < 0 || nchan < 0 || nbands < 1) return fail(DDF_EINVAL, "negative size or nbands < 1");
+  if (nrow == 0 || nchan == 0) return DDF_OK;
+  if (!uvw || !weights || !freqs || !index) return fail(DDF_EINVAL, "null pointer");
+  if (nbands > 1 && !chan_band) return fail(DDF_EINVAL, "chan_band is required when nbands > 1");
+  if (!(cell > 0)) return fail(DDF_EINVAL, "cell must be positive");
+  hipLaunchKernelGGL(weights_index_k, dim3(weight_blocks(nrow * int64_t(nchan))), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), uvw, weights, nrow, nchan, freqs, chan_band, cell, npix, npixx,
+                     nbands, xymax, index);
+  HIP_TRY(hipGetLastError());
+  return DDF_OK;
+}
+
+extern "C" int ddf_weights_briggs(double* grid, int nbands, int64_t npix, double robust, void* stream) {
+  if (nbands < 0 || npix < 0) return fail(DDF_EINVAL, "negative size");
+  if (nbands == 0 || npix == 0) return DDF_OK;
+  if (!grid) return fail(DDF_EINVAL, "null pointer");
+  // numeratorSqrt = 5 * 10 ** (-Robust); its square as Python's float ** 2 (an exact square, rounded)
+  const double num = 5.0 * std::pow(10.0, -robust);
+  const double num2 = num * num;
+  hipLaunchKernelGGL(weights_briggs_k, dim3(unsigned(nbands)), dim3(1024), 0, static_cast<hipStream_t>(stream), grid,
+                     npix, num2);
+  HIP_TRY(hipGetLastError());
+  return DDF_OK;
+}
+
+extern "C" int ddf_weights_finalize(float* weights, const int32_t* index, int64_t n, const double* grid,
+                                    int64_t grid_len, void* stream) {
+  if (n < 0 || grid_len < 0) return fail(DDF_EINVAL, "negative size");
+  if (n == 0) return DDF_OK;
+  if (!weights || !index || !grid || grid_len == 0) return fail(DDF_EINVAL, "null pointer or empty grid");
+  hipLaunchKernelGGL(weights_finalize_k, dim3(weight_blocks(n)), dim3(256), 0, static_cast<hipStream_t>(stream), weights,
+                     index, n, grid, grid_len);
+  HIP_TRY(hipGetLastError());
+  return DDF_OK;
+}
+
 // ======================================================================================================
 // Batched plan (host side).
 // ======================================================================================================

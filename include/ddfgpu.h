@@ -157,6 +157,34 @@ int ddf_degrid(const ddf_grid_desc* g, const ddf_vis* v, const ddf_facet* f, voi
 int ddf_accumulate_weights(double* grid, int64_t grid_len, const float* weights,
                            const int64_t* index, int64_t n, void* stream);
 
+/* ---- imaging weights (uniform / Briggs) ------------------------------------------------------
+ * The weighting steps of ClassVisServer._CalcWeights_serial for one chunk, on the device: cell
+ * index, scatter-add (ddf_accumulate_weights, with the index widened to int64), Briggs transform of
+ * the weight grid, renormalisation.  Grid geometry as the reference sets it
+ * (ClassVisServer.py:1122-1133): xymax = floor(uvmax / cell) + 1, npixx = 2 xymax + 1,
+ * npix = npixx (xymax + 1), grid float64 [nbands][npix]. */
+
+/* Replaces ClassVisServer._uv_to_index (ClassVisServer.py:983-1004): for row r and channel c,
+ * (u, v) = uvw[r][0:2] negated when v < 0; x = floor(u freqs[c] / c0 / cell) + xymax,
+ * y = floor(v freqs[c] / c0 / cell) (c0 = 299792458); index = y npixx + x, plus chan_band[c] npix
+ * when nbands > 1, narrowed to int32 as numpy does; index = 0 where weights[r][c] == 0.
+ * uvw float64 [nrow][3], weights float32 [nrow][nchan], freqs float64 [nchan], chan_band int32
+ * [nchan] (unused when nbands == 1), index int32 [nrow][nchan] (output). */
+int ddf_weights_index(const double* uvw, const float* weights, int64_t nrow, int nchan, const double* freqs,
+                      const int32_t* chan_band, double cell, int64_t npix, int64_t npixx, int nbands,
+                      int64_t xymax, int32_t* index, void* stream);
+
+/* Replaces the robust-weighting grid transform (ClassVisServer.py:1188-1197): per band,
+ * avgW = sum g^2 / sum g and g <- 1 + g (5 10^-robust)^2 / avgW.  grid float64 [nbands][npix]. */
+int ddf_weights_briggs(double* grid, int nbands, int64_t npix, double robust, void* stream);
+
+/* Replaces the renormalisation of ClassVisServer._finalizeWeights_handler
+ * (ClassVisServer.py:1035-1040): weights[i] /= grid[index[i]] with indices >= grid_len set to 0
+ * (negative ones count from the end, as numpy indexing).  weights float32 (in place), index int32,
+ * grid float64 [grid_len]. */
+int ddf_weights_finalize(float* weights, const int32_t* index, int64_t n, const double* grid, int64_t grid_len,
+                         void* stream);
+
 /* ---- batched major cycle: every facet of a chunk in a few persistent launches ----------------
  * Replaces the per-facet job loops of ClassFacetMachine (DDFacet/Imager/ClassFacetMachine.py:
  * 1574-1592 gridding, 1912-1943 degridding, one APP job per facet) and the row-semaphore
