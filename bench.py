@@ -398,11 +398,11 @@ def run_weights(a):
         out = run()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / a.steps
-    # algorithmic bytes: index (uvw per row, weight in, index out), widening to int64, scatter-add
-    # (weight, index, FP64 read-modify-write of the cell), robust transform (three passes over the
-    # grid), renormalisation (weight, index, cell gather, weight out), the weight copy
+    # algorithmic bytes: the weight copy (8 per visibility), index (uvw per row; weight in, index
+    # out), scatter-add (weight, index; an FP64 read-modify-write per cell), robust transform (three
+    # passes over the grid), renormalisation (weight, index, cell gather, weight out)
     nvis = R * Cn
-    nbytes = 24 * R + nvis * (4 + 4 + 4 + 8 + 4 + 8 + 16 + 4 + 4 + 8 + 4 + 8) + 3 * 8 * npix
+    nbytes = 24 * R + nvis * (8 + 8 + 8 + 20) + npix * (16 + 3 * 8)
     ach = nbytes / (ms * 1e-3) / 1e9
     res = {
         "metric": "vis/sec Briggs imaging weights, 81-facet MeerKAT-like chunk",

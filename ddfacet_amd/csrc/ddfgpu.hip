@@ -934,17 +934,95 @@ __global__ __launch_bounds__(256) void weights_index_k(const double* uvw, const 
   }
 }
 
-// One workgroup per band: avgW = sum g^2 / sum g over the band's cells, then g <- 1 + g sSq with
-// sSq = num2 / avgW (ClassVisServer.py:1188-1197), the product rounded before the sum.
-__global__ __launch_bounds__(1024) void weights_briggs_k(double* grid, int64_t npix, double num2) {
-#pragma clang fp contract(off)
-  __shared__ double s2[1024], s1[1024];
-  double* g = grid + int64_t(blockIdx.x) * npix;
+// Scatter-add of a [nrow][nchan] weight array through its int32 cell index (the sum of
+// pyAccumulateWeightsOntoGrid, GridderSmearPols.cc:32-74, zero weights skipped).  A wavefront takes
+// one channel of 64 consecutive rows: rows of one baseline at successive times mostly fall in one
+// cell, so the lanes first sum each run of equal cells (segmented scan over the wavefront) and only
+// a run's last lane issues the FP64 atomic.
+__global__ __launch_bounds__(256) void weights_accumulate_k(double* grid, int64_t glen, const float* w,
+                                                            const int32_t* idx, int64_t nrow, int nchan) {
+  const int lane = threadIdx.x & 63;
+  const int64_t ntile = (nrow + 63) / 64;
+  const int64_t ntask = ntile * nchan;
+  const int64_t wave0 = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwave = (int64_t(gridDim.x) * blockDim.x) >> 6;
+  for (int64_t task = wave0; task < ntask; task += nwave) {
+    const int c = int(task % nchan);
+    const int64_t r = (task / nchan) * 64 + lane;
+    int k = -1;
+    double x = 0;
+    if (r < nrow) {
+      const int64_t i = r * nchan + c;
+      const float wi = w[i];
+      const int ki = idx[i];
+      if (wi != 0.f && ki >= 0 && int64_t(ki) < glen) {
+        k = ki;
+        x = double(wi);
+      }
+    }
+    // head of this lane's run of equal cells
+    const int kp = __shfl_up(k, 1, 64);
+    int start = (lane == 0 || kp != k) ? lane : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int o = __shfl_up(start, d, 64);
+      if (lane >= d && o > start) start = o;
+    }
+    // sum from the run's head to this lane
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const double y = __shfl_up(x, d, 64);
+      if (lane - d >= start) x += y;
+    }
+    const int kn = __shfl_down(k, 1, 64);
+    const bool tail = lane == 63 || kn != k;
+    if (tail && k >= 0) atomicAdd(&grid[k], x);
+  }
+}
+
+// Robust transform of the weight grid (ClassVisServer.py:1188-1197) in two launches over
+// kBriggsBlocks workgroups per band: the first writes each workgroup's partial sums of g^2 and g
+// over a strided share of the band's cells; the second has every workgroup reduce the band's
+// partials in the same order (so each obtains the same avgW = sum g^2 / sum g), then apply
+// g <- 1 + g sSq, sSq = num2 / avgW, to its share, the product rounded before the sum.
+constexpr int kBriggsBlocks = 256;
+
+__global__ __launch_bounds__(256) void weights_briggs_sums_k(const double* grid, int64_t npix, double* part) {
+  __shared__ double s2[256], s1[256];
+  const int band = blockIdx.y;
+  const double* g = grid + int64_t(band) * npix;
   double a2 = 0, a1 = 0;
-  for (int64_t i = threadIdx.x; i < npix; i += blockDim.x) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < npix; i += int64_t(gridDim.x) * blockDim.x) {
     const double x = g[i];
-    a2 += x * x;
+    a2 = fma(x, x, a2);
     a1 += x;
+  }
+  s2[threadIdx.x] = a2;
+  s1[threadIdx.x] = a1;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (int(threadIdx.x) < o) {
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[(int64_t(band) * gridDim.x + blockIdx.x) * 2] = s2[0];
+    part[(int64_t(band) * gridDim.x + blockIdx.x) * 2 + 1] = s1[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void weights_briggs_apply_k(double* grid, int64_t npix, const double* part,
+                                                              double num2) {
+#pragma clang fp contract(off)
+  __shared__ double s2[256], s1[256];
+  const int band = blockIdx.y;
+  const double* pb = part + int64_t(band) * gridDim.x * 2;
+  double a2 = 0, a1 = 0;
+  for (int k = threadIdx.x; k < int(gridDim.x); k += blockDim.x) {
+    a2 += pb[2 * k];
+    a1 += pb[2 * k + 1];
   }
   s2[threadIdx.x] = a2;
   s1[threadIdx.x] = a1;
@@ -958,7 +1036,8 @@ __global__ __launch_bounds__(1024) void weights_briggs_k(double* grid, int64_t n
   }
   const double avg = s2[0] / s1[0];
   const double ssq = num2 / avg;
-  for (int64_t i = threadIdx.x; i < npix; i += blockDim.x) {
+  double* g = grid + int64_t(band) * npix;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < npix; i += int64_t(gridDim.x) * blockDim.x) {
     const double t = g[i] * ssq;
     g[i] = 1. + t;
   }
@@ -1000,15 +1079,39 @@ extern "C" int ddf_weights_index(const double* uvw, const float* weights, int64_
   return DDF_OK;
 }
 
-extern "C" int ddf_weights_briggs(double* grid, int nbands, int64_t npix, double robust, void* stream) {
+extern "C" int ddf_weights_accumulate(double* grid, int64_t grid_len, const float* weights, const int32_t* index,
+                                      int64_t nrow, int nchan, void* stream) {
+  if (nrow < 0 || nchan < 0 || grid_len < 0) return fail(DDF_EINVAL, "negative size");
+  if (nrow == 0 || nchan == 0) return DDF_OK;
+  if (!grid || !weights || !index) return fail(DDF_EINVAL, "null pointer");
+  const int64_t waves = ((nrow + 63) / 64) * nchan;
+  int64_t nb = (waves + 3) / 4;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(weights_accumulate_k, dim3(unsigned(nb)), dim3(256), 0, static_cast<hipStream_t>(stream), grid,
+                     grid_len, weights, index, nrow, nchan);
+  HIP_TRY(hipGetLastError());
+  return DDF_OK;
+}
+
+extern "C" int64_t ddf_weights_briggs_work_size(int nbands) {
+  return nbands > 0 ? int64_t(nbands) * kBriggsBlocks * 2 : 0;
+}
+
+extern "C" int ddf_weights_briggs(double* grid, int nbands, int64_t npix, double robust, double* work,
+                                  int64_t work_len, void* stream) {
   if (nbands < 0 || npix < 0) return fail(DDF_EINVAL, "negative size");
   if (nbands == 0 || npix == 0) return DDF_OK;
-  if (!grid) return fail(DDF_EINVAL, "null pointer");
+  if (nbands > 65535) return fail(DDF_EINVAL, "more than 65535 bands");
+  if (!grid || !work) return fail(DDF_EINVAL, "null pointer");
+  if (work_len < ddf_weights_briggs_work_size(nbands)) return fail(DDF_EINVAL, "workspace too small");
   // numeratorSqrt = 5 * 10 ** (-Robust); its square as Python's float ** 2 (an exact square, rounded)
   const double num = 5.0 * std::pow(10.0, -robust);
   const double num2 = num * num;
-  hipLaunchKernelGGL(weights_briggs_k, dim3(unsigned(nbands)), dim3(1024), 0, static_cast<hipStream_t>(stream), grid,
-                     npix, num2);
+  const dim3 grid2(kBriggsBlocks, unsigned(nbands));
+  hipLaunchKernelGGL(weights_briggs_sums_k, grid2, dim3(256), 0, static_cast<hipStream_t>(stream), grid, npix, work);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(weights_briggs_apply_k, grid2, dim3(256), 0, static_cast<hipStream_t>(stream), grid, npix, work,
+                     num2);
   HIP_TRY(hipGetLastError());
   return DDF_OK;
 }

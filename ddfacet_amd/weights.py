@@ -3,7 +3,8 @@
 Mirrors ``ClassVisServer._CalcWeights_serial`` (DDFacet/Data/ClassVisServer.py:1068-1215) for the
 arrays of one chunk: the weight-grid geometry (:1122-1133), the uv-grid index of every visibility
 (``_uv_to_index``, :983-1004), the scatter-add of the weights onto the grid
-(``_accumulateWeights_handler`` through ``pyAccumulateWeightsOntoGrid``, :1006-1019), the robust
+(``_accumulateWeights_handler`` through ``pyAccumulateWeightsOntoGrid``, :1006-1019, here
+``ddf_weights_accumulate`` on the int32 index), the robust
 transform of the grid (:1188-1197) and the renormalisation (``_finalizeWeights_handler``,
 :1028-1040).  Every step is a kernel of libddfgpu.so; torch provides the device buffers.
 """
@@ -76,10 +77,12 @@ def imaging_weights(uvw, weights, freqs, cell, uvmax, weighting="briggs", robust
     check(L.ddf_weights_index(_ptr(u), _ptr(w), nrow, nchan, _ptr(f), _ptr(cb) if cb is not None else None,
                               float(cell), npix, npixx, nbands, xymax, _ptr(index), C.c_void_p(st)))
     grid = torch.zeros((nbands, npix), dtype=torch.float64, device=dev)
-    # pyAccumulateWeightsOntoGrid takes an int64 index (the int32 index array is widened)
-    idx64 = index.to(torch.int64)
-    check(L.ddf_accumulate_weights(_ptr(grid), grid.numel(), _ptr(w), _ptr(idx64), w.numel(), C.c_void_p(st)))
+    check(L.ddf_weights_accumulate(_ptr(grid), grid.numel(), _ptr(w), _ptr(index), nrow, nchan, C.c_void_p(st)))
     if weighting in ("briggs", "robust"):
-        check(L.ddf_weights_briggs(_ptr(grid), nbands, npix, float(robust), C.c_void_p(st)))
+        L.ddf_weights_briggs_work_size.argtypes = [C.c_int]
+        L.ddf_weights_briggs_work_size.restype = C.c_int64
+        work = torch.empty(int(L.ddf_weights_briggs_work_size(nbands)), dtype=torch.float64, device=dev)
+        check(L.ddf_weights_briggs(_ptr(grid), nbands, npix, float(robust), _ptr(work), work.numel(),
+                                   C.c_void_p(st)))
     check(L.ddf_weights_finalize(_ptr(w), _ptr(index), w.numel(), _ptr(grid), grid.numel(), C.c_void_p(st)))
     return (w, grid) if return_grid else w

@@ -159,8 +159,8 @@ int ddf_accumulate_weights(double* grid, int64_t grid_len, const float* weights,
 
 /* ---- imaging weights (uniform / Briggs) ------------------------------------------------------
  * The weighting steps of ClassVisServer._CalcWeights_serial for one chunk, on the device: cell
- * index, scatter-add (ddf_accumulate_weights, with the index widened to int64), Briggs transform of
- * the weight grid, renormalisation.  Grid geometry as the reference sets it
+ * index, scatter-add (ddf_weights_accumulate), Briggs transform of the weight grid,
+ * renormalisation.  Grid geometry as the reference sets it
  * (ClassVisServer.py:1122-1133): xymax = floor(uvmax / cell) + 1, npixx = 2 xymax + 1,
  * npix = npixx (xymax + 1), grid float64 [nbands][npix]. */
 
@@ -174,9 +174,20 @@ int ddf_weights_index(const double* uvw, const float* weights, int64_t nrow, int
                       const int32_t* chan_band, double cell, int64_t npix, int64_t npixx, int nbands,
                       int64_t xymax, int32_t* index, void* stream);
 
+/* The scatter-add of _accumulateWeights_handler (ClassVisServer.py:1006-1019) for a [nrow][nchan]
+ * weight array and its int32 index from ddf_weights_index: grid[index[r][c]] += weights[r][c] for
+ * non-zero weights (pyAccumulateWeightsOntoGrid, GridderSmearPols.cc:32-74, without the widening
+ * of the index to int64); indices outside [0, grid_len) are skipped. */
+int ddf_weights_accumulate(double* grid, int64_t grid_len, const float* weights, const int32_t* index, int64_t nrow,
+                           int nchan, void* stream);
+
 /* Replaces the robust-weighting grid transform (ClassVisServer.py:1188-1197): per band,
- * avgW = sum g^2 / sum g and g <- 1 + g (5 10^-robust)^2 / avgW.  grid float64 [nbands][npix]. */
-int ddf_weights_briggs(double* grid, int nbands, int64_t npix, double robust, void* stream);
+ * avgW = sum g^2 / sum g and g <- 1 + g (5 10^-robust)^2 / avgW.  grid float64 [nbands][npix];
+ * work: float64 scratch of ddf_weights_briggs_work_size(nbands) elements (per-workgroup partial
+ * sums), work_len its length. */
+int64_t ddf_weights_briggs_work_size(int nbands);
+int ddf_weights_briggs(double* grid, int nbands, int64_t npix, double robust, double* work, int64_t work_len,
+                       void* stream);
 
 /* Replaces the renormalisation of ClassVisServer._finalizeWeights_handler
  * (ClassVisServer.py:1035-1040): weights[i] /= grid[index[i]] with indices >= grid_len set to 0
