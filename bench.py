@@ -360,6 +360,61 @@ def run_autograd(a, gd_steps=20):
     print(json.dumps(out))
 
 
+def run_flags(a):
+    """Flag preparation of the 81-facet workload's chunk on the device (autocorrelations, NaNs, 1e9
+    fill, equalisation over the 4 correlations), chunk resident in HBM.  CPU baseline: the numpy
+    restatement (flagprep.prepare_flags) on every 8th row."""
+    import torch
+    from ddfacet_amd import configs
+    from ddfacet_amd.flagprep import prepare_flags, prepare_flags_device
+    P = configs.cfg2(with_vis=False, with_jones=False, log=log)
+    R, Cn, K = P.flags.shape
+    rng = np.random.default_rng(77)
+    cpu = None
+    if not a.no_cpu:
+        sl = slice(0, R, 8)
+        d = (rng.standard_normal((P.flags[sl].shape)) + 1j * rng.standard_normal(P.flags[sl].shape)).astype(np.complex64)
+        f = P.flags[sl].copy()
+        t0 = time.perf_counter()
+        prepare_flags(f, P.a0[sl], P.a1[sl], d)
+        dt = time.perf_counter() - t0
+        cpu = {"value": f.shape[0] * Cn / dt, "unit": "vis/s", "cores": 1, "kind": "port",
+               "sample": "numpy restatement (flagprep.prepare_flags) on every 8th row (%d visibilities), %.1f s"
+                         % (f.shape[0] * Cn, dt)}
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    flags = torch.from_numpy(P.flags.astype(np.uint8)).to(dev)
+    gen = torch.Generator(device=dev).manual_seed(78)
+    data = torch.randn((R, Cn, K), dtype=torch.complex64, device=dev, generator=gen)
+    ta0 = torch.from_numpy(P.a0.astype(np.int32)).to(dev)
+    ta1 = torch.from_numpy(P.a1.astype(np.int32)).to(dev)
+    for _ in range(a.warmup):
+        prepare_flags_device(flags, ta0, ta1, data)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        prepare_flags_device(flags, ta0, ta1, data)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    nvis = R * Cn
+    # algorithmic bytes: flags read and written (1 byte per correlation each), data read (8 per
+    # correlation), antennas (8 per row); the 1e9 writes of flagged values are not counted
+    nbytes = nvis * K * (1 + 1 + 8) + 8 * R
+    ach = nbytes / (ms * 1e-3) / 1e9
+    res = {
+        "metric": "vis/sec chunk flag preparation, 81-facet MeerKAT-like chunk",
+        "value": nvis / (ms * 1e-3), "unit": "vis/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8 flags, c64 data",
+        "data": "synthetic",
+        "config": {"workload": "flag preparation of the 81-facet MeerKAT-like chunk (64 antennas, 600 x 8 s, 64 ch, "
+                               "4 corr)", "rows": R, "channels": Cn, "nvis": nvis},
+        "roofline": {"bound": "hbm", "kernel": "prepare_flags_k", "achieved": ach, "peak": 8000.0, "unit": "GB/s",
+                     "frac": ach / 8000.0, "traffic": None, "algorithmic_bytes_per_launch": nbytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res))
+
+
 def run_weights(a):
     """Briggs imaging weights (robust 0) of the 81-facet workload's chunk on the device: uv-grid
     index, scatter-add, robust transform, renormalisation, inputs resident in HBM.  CPU baseline:
@@ -428,13 +483,13 @@ def main():
     ap.add_argument("--mode", default="batched", choices=["batched", "plain"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--facets", type=int, default=0, help="use only the first N facets (debug)")
-    ap.add_argument("--workload", default="meerkat81", choices=["meerkat81", "uneven121", "autograd25", "weights"],
+    ap.add_argument("--workload", default="meerkat81", choices=["meerkat81", "uneven121", "autograd25", "weights", "flags"],
                     help="meerkat81: the 81-facet MeerKAT-like cycle (the headline line); uneven121: "
                          "121 facets x 4 bands with baseline-dependent averaging, plain per-facet "
                          "launches against the autotuned batched cycle (eager and graph-replayed); "
                          "autograd25: 20 chi^2 gradient-descent steps through the differentiable "
                          "degridder on 25 facets x 4 bands; weights: Briggs imaging weights of the "
-                         "81-facet workload's chunk")
+                         "81-facet workload's chunk; flags: flag preparation of that chunk")
     ap.add_argument("--tile", type=int, default=0, help="uv tile edge of the batched plan (0: the plan default)")
     ap.add_argument("--blocks-per-item", type=int, default=0,
                     help="BDA blocks per pass-A work item of the batched plan (0: the plan default)")
@@ -470,6 +525,8 @@ def main():
         return run_autograd(a)
     if a.workload == "weights":
         return run_weights(a)
+    if a.workload == "flags":
+        return run_flags(a)
     from ddfacet_amd import configs
     t0 = time.time()
     # geometry, maps, CFs and Jones on the host; the CPU baseline runs before the GPU is touched

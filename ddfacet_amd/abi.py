@@ -174,6 +174,7 @@ PRECISION = {"mixed": 0, "fp64": 1, "fp32": 2}
 EXTRA_SIGNATURES = {
     "ddf_weights_index": [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_int64,
                           C.c_int64, C.c_int, C.c_int64, C.c_void_p, C.c_void_p],
+    "ddf_prepare_flags": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_void_p],
     "ddf_weights_accumulate": [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p],
     "ddf_weights_briggs": [C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_void_p, C.c_int64, C.c_void_p],
     "ddf_weights_finalize": [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p],

@@ -1127,6 +1127,47 @@ extern "C" int ddf_weights_finalize(float* weights, const int32_t* index, int64_
   return DDF_OK;
 }
 
+// ---- chunk flag preparation (ClassMS.py:1155-1167) -------------------------------------------------
+namespace {
+// One thread per (row, channel): autocorrelation rows flagged, NaN visibilities (either part)
+// flagged, the values flagged so far overwritten with 1e9 + 0i, then every correlation flagged when
+// any is, in the reference's order.
+__global__ __launch_bounds__(256) void prepare_flags_k(uint8_t* flags, float2* data, const int32_t* a0,
+                                                       const int32_t* a1, int64_t nrow, int nchan, int ncorr) {
+  const int64_t n = nrow * int64_t(nchan);
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r = i / nchan;
+    const bool autoc = a0[r] == a1[r];
+    bool any = false;
+    for (int k = 0; k < ncorr; ++k) {
+      const int64_t j = i * ncorr + k;
+      bool f = flags[j] != 0 || autoc;
+      if (data) {
+        const float2 x = data[j];
+        f = f || isnan(x.x) || isnan(x.y);
+        if (f) data[j] = make_float2(1e9f, 0.f);
+      }
+      any = any || f;
+      flags[j] = f ? 1 : 0;
+    }
+    if (any)
+      for (int k = 0; k < ncorr; ++k) flags[i * ncorr + k] = 1;
+  }
+}
+}  // namespace
+
+extern "C" int ddf_prepare_flags(uint8_t* flags, float* data, const int32_t* a0, const int32_t* a1, int64_t nrow,
+                                 int nchan, int ncorr, void* stream) {
+  if (nrow < 0 || nchan < 0 || ncorr < 1 || ncorr > 4) return fail(DDF_EINVAL, "bad sizes (ncorr must be 1..4)");
+  if (nrow == 0 || nchan == 0) return DDF_OK;
+  if (!flags || !a0 || !a1) return fail(DDF_EINVAL, "null pointer");
+  hipLaunchKernelGGL(prepare_flags_k, dim3(weight_blocks(nrow * int64_t(nchan))), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), flags, reinterpret_cast<float2*>(data), a0, a1, nrow, nchan,
+                     ncorr);
+  HIP_TRY(hipGetLastError());
+  return DDF_OK;
+}
+
 // ======================================================================================================
 // Batched plan (host side).
 // ======================================================================================================

@@ -196,6 +196,16 @@ int ddf_weights_briggs(double* grid, int nbands, int64_t npix, double robust, do
 int ddf_weights_finalize(float* weights, const int32_t* index, int64_t n, const double* grid, int64_t grid_len,
                          void* stream);
 
+/* ---- chunk flag preparation ------------------------------------------------------------------
+ * Replaces the flag preparation of the chunk reader (ClassMS.ReadData, DDFacet/Data/ClassMS.py:
+ * 1155-1167), which the gridders rely on (they test the flag of correlation 0 only): rows with
+ * a0 == a1 flagged; visibilities with a NaN part flagged; every visibility flagged so far set to
+ * 1e9 + 0i; then all correlations of a (row, channel) flagged when any is.  flags uint8 (0 / 1)
+ * [nrow][nchan][ncorr] and data complex64 (interleaved re, im) [nrow][nchan][ncorr], both in place;
+ * data may be null (flags only); a0, a1 int32 [nrow]; ncorr 1..4. */
+int ddf_prepare_flags(uint8_t* flags, float* data, const int32_t* a0, const int32_t* a1, int64_t nrow, int nchan,
+                      int ncorr, void* stream);
+
 /* ---- batched major cycle: every facet of a chunk in a few persistent launches ----------------
  * Replaces the per-facet job loops of ClassFacetMachine (DDFacet/Imager/ClassFacetMachine.py:
  * 1574-1592 gridding, 1912-1943 degridding, one APP job per facet) and the row-semaphore
