@@ -1133,8 +1133,43 @@ namespace {
 // flagged, the values flagged so far overwritten with 1e9 + 0i, then every correlation flagged when
 // any is, in the reference's order.
 __global__ __launch_bounds__(256) void prepare_flags_k(uint8_t* flags, float2* data, const int32_t* a0,
-                                                       const int32_t* a1, int64_t nrow, int nchan, int ncorr) {
+                                                       const int32_t* a1, int64_t nrow, int nchan, int ncorr,
+                                                       int vec4) {
   const int64_t n = nrow * int64_t(nchan);
+  if (vec4) {
+    // four correlations with 4-byte aligned flags and 16-byte aligned values: the flags of a
+    // (row, channel) as one 32-bit word, the values as two 16-byte loads
+    uint32_t* fw = reinterpret_cast<uint32_t*>(flags);
+    float4* d4 = reinterpret_cast<float4*>(data);
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+      const int64_t r = i / nchan;
+      const bool autoc = a0[r] == a1[r];
+      const uint32_t f0 = fw[i];
+      bool f[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) f[k] = ((f0 >> (8 * k)) & 0xFFu) != 0 || autoc;
+      if (data) {
+        float4 x0 = d4[2 * i], x1 = d4[2 * i + 1];
+        f[0] = f[0] || isnan(x0.x) || isnan(x0.y);
+        f[1] = f[1] || isnan(x0.z) || isnan(x0.w);
+        f[2] = f[2] || isnan(x1.x) || isnan(x1.y);
+        f[3] = f[3] || isnan(x1.z) || isnan(x1.w);
+        if (f[0] || f[1]) {
+          if (f[0]) { x0.x = 1e9f; x0.y = 0.f; }
+          if (f[1]) { x0.z = 1e9f; x0.w = 0.f; }
+          d4[2 * i] = x0;
+        }
+        if (f[2] || f[3]) {
+          if (f[2]) { x1.x = 1e9f; x1.y = 0.f; }
+          if (f[3]) { x1.z = 1e9f; x1.w = 0.f; }
+          d4[2 * i + 1] = x1;
+        }
+      }
+      const bool any = f[0] || f[1] || f[2] || f[3];
+      fw[i] = any ? 0x01010101u : 0u;
+    }
+    return;
+  }
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
     const int64_t r = i / nchan;
     const bool autoc = a0[r] == a1[r];
@@ -1161,9 +1196,11 @@ extern "C" int ddf_prepare_flags(uint8_t* flags, float* data, const int32_t* a0,
   if (nrow < 0 || nchan < 0 || ncorr < 1 || ncorr > 4) return fail(DDF_EINVAL, "bad sizes (ncorr must be 1..4)");
   if (nrow == 0 || nchan == 0) return DDF_OK;
   if (!flags || !a0 || !a1) return fail(DDF_EINVAL, "null pointer");
+  const int vec4 = ncorr == 4 && reinterpret_cast<uintptr_t>(flags) % 4 == 0 &&
+                   reinterpret_cast<uintptr_t>(data) % 16 == 0;
   hipLaunchKernelGGL(prepare_flags_k, dim3(weight_blocks(nrow * int64_t(nchan))), dim3(256), 0,
                      static_cast<hipStream_t>(stream), flags, reinterpret_cast<float2*>(data), a0, a1, nrow, nchan,
-                     ncorr);
+                     ncorr, vec4);
   HIP_TRY(hipGetLastError());
   return DDF_OK;
 }

@@ -37,3 +37,22 @@ def test_prepare_flags_device_matches_host(ncorr, with_data):
     if with_data:
         got = td.cpu().numpy()
         assert np.array_equal(got.view(np.uint32), data.view(np.uint32))
+
+
+def test_prepare_flags_device_on_unaligned_views():
+    """Flags and values that start one (row, channel) into a buffer (values 8-byte but not 16-byte
+    aligned): the word / 16-byte path is not taken, the result is the same."""
+    from ddfacet_amd.flagprep import prepare_flags, prepare_flags_device
+    flags, data, a0, a1 = _chunk(11, nrow=300, nchan=5, ncorr=4)
+    dev = torch.device("cuda")
+    bf = torch.zeros(flags.size + 4, dtype=torch.uint8, device=dev)
+    bd = torch.zeros(data.size + 1, dtype=torch.complex64, device=dev)
+    tf = bf[1:1 + flags.size].view(flags.shape)
+    td = bd[1:1 + data.size].view(data.shape)
+    tf.copy_(torch.from_numpy(flags.astype(np.uint8)))
+    td.copy_(torch.from_numpy(data))
+    prepare_flags_device(tf, torch.from_numpy(a0).to(dev), torch.from_numpy(a1).to(dev), td)
+    torch.cuda.synchronize()
+    want_f = prepare_flags(flags.copy(), a0, a1, data)
+    assert np.array_equal(tf.cpu().numpy().astype(bool), want_f)
+    assert np.array_equal(td.cpu().numpy().view(np.uint32), data.view(np.uint32))
