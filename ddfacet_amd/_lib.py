@@ -37,6 +37,8 @@ def lib():
     L.ddf_accumulate_weights.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
     L.ddf_accumulate_weights.restype = C.c_int
     _bind_optional(L)
+    L.ddf_weights_briggs_work_size.argtypes = [C.c_int]
+    L.ddf_weights_briggs_work_size.restype = C.c_int64
     if L.ddf_version() != abi.DDF_ABI_VERSION:
         raise DDFError("libddfgpu.so ABI %d, expected %d" % (L.ddf_version(), abi.DDF_ABI_VERSION))
     _LIB = L

@@ -79,8 +79,6 @@ def imaging_weights(uvw, weights, freqs, cell, uvmax, weighting="briggs", robust
     grid = torch.zeros((nbands, npix), dtype=torch.float64, device=dev)
     check(L.ddf_weights_accumulate(_ptr(grid), grid.numel(), _ptr(w), _ptr(index), nrow, nchan, C.c_void_p(st)))
     if weighting in ("briggs", "robust"):
-        L.ddf_weights_briggs_work_size.argtypes = [C.c_int]
-        L.ddf_weights_briggs_work_size.restype = C.c_int64
         work = torch.empty(int(L.ddf_weights_briggs_work_size(nbands)), dtype=torch.float64, device=dev)
         check(L.ddf_weights_briggs(_ptr(grid), nbands, npix, float(robust), _ptr(work), work.numel(),
                                    C.c_void_p(st)))
