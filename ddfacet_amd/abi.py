@@ -184,6 +184,7 @@ EXTRA_SIGNATURES = {
     "ddf_plan_grid": [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p],
     "ddf_plan_degrid": [C.c_void_p, C.c_void_p, C.c_void_p],
     "ddf_plan_info": [C.c_void_p, C.POINTER(C.c_int64), C.c_int32],
+    "ddf_plan_placements": [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p],
     "ddf_cf_build": [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p],
     "ddf_plan_timing": [C.c_void_p, C.c_int32],
     "ddf_plan_pass_ms": [C.c_void_p, C.POINTER(C.c_float), C.c_int32],

@@ -1839,6 +1839,18 @@ extern "C" int ddf_plan_info(const ddf_plan* P, int64_t* out, int32_t n) {
   return DDF_OK;
 }
 
+extern "C" int ddf_plan_placements(const ddf_plan* P, int32_t degrid, uint64_t* out, int64_t n, void* stream) {
+  if (!P || !out) return fail(DDF_EINVAL, "null argument");
+  const BatchParams& p = degrid ? P->pd : P->pg;
+  const int64_t nb = degrid ? P->vd.nblocks : P->vg.nblocks;
+  const int64_t need = nb * int64_t(p.nf);
+  if (n < need) return fail(DDF_EINVAL, "output holds fewer than nblocks * nfacets records");
+  if (need == 0) return DDF_OK;
+  HIP_TRY(hipMemcpyAsync(out, degrid ? P->place_d : P->place_g, size_t(need) * sizeof(uint64_t),
+                         hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)));
+  return DDF_OK;
+}
+
 extern "C" int ddf_plan_timing(ddf_plan* P, int32_t enable) {
   if (!P) return fail(DDF_EINVAL, "null plan");
   if (enable && !P->ev[0][0]) {

@@ -217,6 +217,29 @@ class DeviceFacetSet:
                 "tile", "facet_group", "grid_tile_workgroups", "degrid_tile_workgroups", "adjoint_tile_workgroups"]
         return dict(zip(keys, list(a)))
 
+    def placements(self, degrid=False, stream=None):
+        """The plan's integer placement maps (ddf_plan_placements) decoded to int32
+        [nfacets, nblocks, 8] rows [valid, iw, conj (W <= 0), locx, locy, io, jo, band]."""
+        nb = self.nblocks_degrid if degrid else self.nblocks_grid
+        raw = torch.empty((nb, self.nf), dtype=torch.int64, device=self.dev)
+        check(lib().ddf_plan_placements(self._plan, int(bool(degrid)), C.c_void_p(raw.data_ptr()), raw.numel(),
+                                        C.c_void_p(stream_handle(stream))))
+        torch.cuda.synchronize(self.dev)
+        r = raw.cpu().numpy().view(np.uint64).T
+        nw = self.cf[0]["nw"]
+        valid = (r >> np.uint64(63)).astype(np.int32)
+        plane = ((r >> np.uint64(32)) & np.uint64(0xFF)).astype(np.int32)
+        # plane in put order: the degrid map serves W > 0 from the put-order conjugate list
+        conj = (plane >= nw) if not degrid else (plane < nw)
+        out = np.stack([valid, plane % nw, conj.astype(np.int32),
+                        (r & np.uint64(0xFFFF)).astype(np.int32),
+                        ((r >> np.uint64(16)) & np.uint64(0xFFFF)).astype(np.int32),
+                        ((r >> np.uint64(40)) & np.uint64(0xFF)).astype(np.int32),
+                        ((r >> np.uint64(48)) & np.uint64(0xFF)).astype(np.int32),
+                        ((r >> np.uint64(56)) & np.uint64(0x7F)).astype(np.int32)], axis=-1)
+        out[valid == 0] = 0
+        return out
+
     def plan_timing(self, enable=True):
         """Bracket every batched pass with HIP events on its launch stream (ddf_plan_timing)."""
         check(lib().ddf_plan_timing(self._plan, int(bool(enable))))

@@ -147,18 +147,20 @@ def random_problem(seed, nant=5, ntime=6, nchan=8, ncorr=4, npix=48, cell_arcsec
                    facet_spacing_px=40, nw=3, sups=(5, 7, 7), overs=5, nband=1, npol=1,
                    corr_codes=(9, 10, 11, 12), stokes_codes=(1,), jones=None, decorr=False,
                    bda="mixed", flag_frac=0.1, sparsify=False, freq0=1.2e9, freq1=1.6e9,
-                   equidistant=True, wmax_scale=1.0, sort_by_baseline=False):
+                   equidistant=True, wmax_scale=1.0, sort_by_baseline=False, uv_scale=1.0):
     """A small random problem whose uv coverage fits a `npix` grid (so every code path runs).
 
     jones: None, or dict(mode=1|2, interp=0|1, beam=bool, kms=bool, apply_amp, apply_phase, scale).
     bda: "pervis" (one block per row x chan), "mixed" (random multi-row / multi-chan blocks).
+    uv_scale > 1 widens the array so that the longest baselines fall off the grid; wmax_scale < 1
+    puts the largest |w| beyond the last w-plane (both drop blocks, as the reference does).
     """
     rng = np.random.default_rng(seed)
     cell = cell_arcsec * synth.ARCSEC
     # baselines short enough that |u|/lambda stays inside the grid
     umax_lambda = 0.42 * npix / 2 / (npix * cell)
     fmax = max(freq0, freq1)
-    diam = umax_lambda * C_LIGHT / fmax * 2.2
+    diam = umax_lambda * C_LIGHT / fmax * 2.2 * uv_scale
     if equidistant:
         freqs = np.linspace(freq0, freq1, nchan)
     else:

@@ -660,6 +660,75 @@ int ref_degrid(const ddf_grid_desc* g, const ddf_vis* v, const ddf_facet* f) {
   return degrid_impl(g, v, f, false);
 }
 
+// Integer placement maps of every BDA block for one facet: the block means (gridder.h:211-289 over
+// unflagged visibilities for gridding, degridder.h:141-167 over all rows for degridding) and the
+// placement of gridder.h:321-351 / degridder.h:170-200 with the .5-boundary clamp of `place`.
+// out: int32 [nblocks][8] = valid, iw, conj (W <= 0), locx, locy, io, jo, band.
+int ref_place(const ddf_grid_desc* g, const ddf_vis* v, const ddf_facet* f, int degrid, int32_t* out) {
+  BlockIter bi(v);
+  const double Cu = f->cu, Cv = f->cv;
+  const size_t nrow = size_t(v->nrow), nchan = size_t(v->nchan), ncorr = size_t(v->ncorr);
+  const int32_t* rec = bi.rec;
+  for (int64_t b = 0; b < bi.nblocks; ++b) {
+    int32_t* o = out + 8 * b;
+    for (int k = 0; k < 8; ++k) o[k] = 0;
+    const int nr = bi.sizes[b] - 2;
+    const size_t ch0 = size_t(rec[0]), ch1 = size_t(rec[1]);
+    const int32_t* rows = rec + 2;
+    rec += bi.sizes[b];
+    if (!degrid && v->sparsification && !v->sparsification[b]) continue;
+    double um = 0, vm = 0, wm = 0, fmean = 0;
+    int nv = 0;
+    if (!degrid) {
+      for (int r = 0; r < nr; ++r) {
+        size_t row = size_t(rows[r]);
+        if (row >= nrow) continue;
+        const double* uvw = v->uvw + 3 * row;
+        const double U = uvw[0], V = uvw[1], W = uvw[2];
+        for (size_t c = ch0; c < ch1; ++c) {
+          if (v->flags[(row * nchan + c) * ncorr]) continue;
+          um += U + W * Cu;
+          vm += V + W * Cv;
+          wm += W;
+          fmean += v->freqs[c];
+          ++nv;
+        }
+      }
+      if (nv == 0) continue;
+      um /= nv; vm /= nv; wm /= nv; fmean /= nv;
+    } else {
+      for (size_t c = ch0; c < ch1; ++c) fmean += v->freqs[c];
+      fmean /= double(ch1 - ch0);
+      for (int r = 0; r < nr; ++r) {
+        size_t row = size_t(rows[r]);
+        if (row >= nrow) continue;
+        const double* uvw = v->uvw + 3 * row;
+        um += uvw[0] + uvw[2] * Cu;
+        vm += uvw[1] + uvw[2] * Cv;
+        wm += uvw[2];
+        ++nv;
+      }
+      if (nv == 0) continue;
+      um /= nv; vm /= nv; wm /= nv;
+    }
+    const int gchan = v->chan_map[ch0];
+    if (gchan < 0 || gchan >= g->nband) continue;
+    Place pl;
+    if (!place(g, f->cf, um, vm, wm, fmean / kC, &pl)) continue;
+    const int nw = f->cf.nw, O = f->cf.overs;
+    const int64_t sub = (pl.cfoff - f->cf.offset[pl.plane]) / (int64_t(pl.S) * pl.S);
+    o[0] = 1;
+    o[1] = pl.plane % nw;
+    o[2] = (wm > 0) ? 0 : 1;
+    o[3] = pl.locx;
+    o[4] = pl.locy;
+    o[5] = int32_t(sub / O);
+    o[6] = int32_t(sub % O);
+    o[7] = gchan;
+  }
+  return DDF_OK;
+}
+
 // All facets of one major cycle, one facet per thread, as the reference process pool runs them
 // (DDFacet/Other/AsyncProcessPool.py:831-861; one APP job per facet, ClassFacetMachine.py:1589).
 int ref_grid_facets(const ddf_grid_desc* g, const ddf_vis* v, const ddf_facet* f, int nf, int nthreads) {

@@ -2,7 +2,8 @@
 
 ``grid(*args, f64=False)`` and ``degrid(*args)`` take the reference positional argument lists of
 pyGridderWPol32|64 / pyDeGridderWPol (DDFacet/Gridder/GridderSmearPols.cc:76-97, 212-233) and
-update the numpy outputs in place, exactly like the reference module.
+update the numpy outputs in place, exactly like the reference module.  ``place(*args, degrid)``
+returns the per-block integer placement maps of the same call.
 """
 import ctypes as C
 import os
@@ -31,6 +32,9 @@ def lib():
             fn.argtypes = [C.POINTER(abi.ddf_grid_desc), C.POINTER(abi.ddf_vis), C.POINTER(abi.ddf_facet),
                            C.c_int, C.c_int]
             fn.restype = C.c_int
+        L.ref_place.argtypes = [C.POINTER(abi.ddf_grid_desc), C.POINTER(abi.ddf_vis), C.POINTER(abi.ddf_facet),
+                                C.c_int, C.c_void_p]
+        L.ref_place.restype = C.c_int
         L.ref_last_error.restype = C.c_char_p
         _LIB = L
     return _LIB
@@ -58,6 +62,18 @@ def degrid(*args):
     g, v, f = marshal.fill_structs(spec, host_addr)
     _check(lib().ref_degrid(C.byref(g), C.byref(v), C.byref(f)))
     return spec.vis
+
+
+def place(*args, degrid=False):
+    """Integer placement maps of every BDA block for the facet of a pyGridderWPol (degrid=False)
+    or pyDeGridderWPol (degrid=True) argument list: int32 [nblocks, 8] = valid, iw, conj
+    (W <= 0), locx, locy, io, jo (oversampling sub-kernel), band (gridder.h:321-351,
+    degridder.h:170-200)."""
+    spec = marshal.parse_degrid_args(args) if degrid else marshal.parse_grid_args(args, False)
+    g, v, f = marshal.fill_structs(spec, host_addr)
+    out = np.zeros((max(int(v.nblocks), 1), 8), np.int32)
+    _check(lib().ref_place(C.byref(g), C.byref(v), C.byref(f), int(bool(degrid)), out.ctypes.data))
+    return out[:int(v.nblocks)]
 
 
 def grid_facets(specs, nthreads):

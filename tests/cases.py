@@ -4,7 +4,9 @@ Each case is (name, kwargs of ddfacet_amd.problems.random_problem).  Together th
 reference code paths: every correlation / Stokes dispatch family, Jones modes 1 and 2 with DD
 and beam tables, interpolation mode 1, decorrelation, several bands and polarisations,
 sparsification, non-equidistant channels, multi-row / multi-channel BDA blocks and one block
-per visibility, several facets (non-zero l0, m0, Cu, Cv), 1/2/4 correlations.
+per visibility, several facets (non-zero l0, m0, Cu, Cv), 1/2/4 correlations, supports
+of 17-23 (stamps of more than 256 cells), and blocks dropped off the grid or beyond the last
+w-plane.
 """
 CASES = [
     ("base", dict(seed=11)),
@@ -27,6 +29,9 @@ CASES = [
     ("jones2_multiband", dict(seed=28, nband=2, jones=dict(mode=2), nchan=12)),
     ("iu_quirk", dict(seed=29, npol=2, stokes_codes=(1, 3))),
     ("qu", dict(seed=30, npol=2, stokes_codes=(2, 3))),
+    ("wide_support", dict(seed=31, nant=6, nfacet=2, nw=4, sups=(17, 19, 21, 23), overs=5, npix=96,
+                          jones=dict(mode=1))),
+    ("drops", dict(seed=32, nant=7, nfacet=2, uv_scale=1.6, wmax_scale=0.6, jones=dict(mode=1))),
 ]
 
 

@@ -8,7 +8,7 @@ pyDeGridderWPol (vis after the subtraction of a seeded random model grid) are st
 digest of the inputs, in tests/golden/<case>.npz.  The inputs themselves are regenerated from
 the seeds by the tests.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]      (default: every case)
 """
 import hashlib
 import os
@@ -63,7 +63,8 @@ def main():
     from oracle import refmod
     mod = refmod.module()
     here = os.path.dirname(os.path.abspath(__file__))
-    for name, _ in CASES:
+    names = sys.argv[1:] or [c[0] for c in CASES]
+    for name in names:
         out = run_case(mod, name)
         np.savez_compressed(os.path.join(here, name + ".npz"), **out)
         print("wrote", name)

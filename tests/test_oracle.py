@@ -42,3 +42,20 @@ def test_golden_cases_exercise_gridding():
         G = _golden(name)
         assert np.count_nonzero(G["grid_f0_psf0"]) > 50, name
         assert np.abs(G["sumwt_f0_psf0"]).sum() > 0, name
+
+
+@pytest.mark.parametrize("name", [c[0] for c in CASES])
+def test_oracle_placement_maps_equal_reference_delta_stamps(name):
+    """The oracle's integer placement maps (w-plane, conjugate choice, grid cell, oversampling
+    sub-kernel, band, dropped blocks) of every block and facet, grid and degrid maps, equal the
+    ones the reference gridder / degridder reveal through delta stamps, bit for bit."""
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    D = np.load(os.path.join(here, "delta_stamps.npz"), allow_pickle=False)
+    P = problem(name)
+    for fi in range(len(P.facets)):
+        g = P.new_grid()
+        got_g = cpuref.place(*P.grid_args(fi, g, np.zeros((P.nband, P.npol))))
+        got_d = cpuref.place(*P.degrid_args(fi, g, P.vis.copy()), degrid=True)
+        np.testing.assert_array_equal(got_g, D[name + "_grid"][fi])
+        np.testing.assert_array_equal(got_d, D[name + "_degrid"][fi])
