@@ -363,10 +363,11 @@ def run_autograd(a, gd_steps=20):
 def run_flags(a):
     """Flag preparation of the 81-facet workload's chunk on the device (autocorrelations, NaNs, 1e9
     fill, equalisation over the 4 correlations), chunk resident in HBM.  CPU baseline: the numpy
-    restatement (flagprep.prepare_flags) on every 8th row."""
+    restatement (oracle/flagprep.py) on every 8th row."""
     import torch
     from ddfacet_amd import configs
-    from ddfacet_amd.flagprep import prepare_flags, prepare_flags_device
+    from ddfacet_amd.flagprep import prepare_flags_device
+    from oracle.flagprep import prepare_flags
     P = configs.cfg2(with_vis=False, with_jones=False, log=log)
     R, Cn, K = P.flags.shape
     rng = np.random.default_rng(77)
@@ -379,7 +380,7 @@ def run_flags(a):
         prepare_flags(f, P.a0[sl], P.a1[sl], d)
         dt = time.perf_counter() - t0
         cpu = {"value": f.shape[0] * Cn / dt, "unit": "vis/s", "cores": 1, "kind": "port",
-               "sample": "numpy restatement (flagprep.prepare_flags) on every 8th row (%d visibilities), %.1f s"
+               "sample": "numpy restatement (oracle/flagprep.py) on every 8th row (%d visibilities), %.1f s"
                          % (f.shape[0] * Cn, dt)}
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)

@@ -4,28 +4,15 @@ The gridder tests the flag of correlation 0 only (gridder.h:229); that is valid 
 reader flags autocorrelations, flags NaN visibilities (and overwrites every flagged value with
 1e9), and then flags all correlations of a (row, channel) when any of them is flagged.
 
-``prepare_flags`` is the host restatement (numpy, once per chunk); ``prepare_flags_device`` runs
-the same steps on a chunk resident in HBM (``ddf_prepare_flags``).
+``prepare_flags_device`` runs these steps on a chunk resident in HBM (``ddf_prepare_flags``): a
+thread per (row, channel) flags autocorrelations and NaNs, fills flagged values with 1e9 and
+equalises the flags over the correlations, in place.
 """
 import ctypes as C
 
-import numpy as np
-
-
-def prepare_flags(flags, a0, a1, data=None):
-    """flags: bool [nrow, nchan, ncorr], updated in place and returned; data: complex
-    [nrow, nchan, ncorr] or None (NaNs flagged, then the values flagged so far set to 1e9, in
-    place, before the equalisation over correlations, in the reference's order)."""
-    flags[np.asarray(a0) == np.asarray(a1)] = True
-    if data is not None:
-        flags[np.isnan(data)] = True
-        data[flags] = 1e9
-    flags[flags.any(axis=2)] = True
-    return flags
-
 
 def prepare_flags_device(flags, a0, a1, data=None, stream=None):
-    """prepare_flags on device tensors, in place: flags uint8 or bool [nrow, nchan, ncorr] (CUDA),
+    """The flag preparation on device tensors, in place: flags uint8 or bool [nrow, nchan, ncorr] (CUDA),
     data complex64 of the same shape or None, a0 / a1 int32 [nrow] (CUDA)."""
     import torch
     from ._lib import check, lib

@@ -1,10 +1,10 @@
 """Facet geometry and the image side of the major cycle on the GPU.
 
-Geometry: a regular nfx x nfx tessellation of an m x m main image (the layout
-ClassFacetMachineTessel produces for a square grid of facets): facet centres ``pixCentral``,
-facet (l0, m0), and the spatial weights ``SW`` -- the facet mask smoothed by a Gaussian of
-sigma ``MixingWidth`` pixels, normalised to 1 and clipped below 1e-3 (ClassFacetMachine.py:
-766-820).  Main-image pixel (X, Y) maps to (l, m) = ((X - m//2) cell, (Y - m//2) cell), the
+Geometry: the regular nfx x nfx tessellation of an m x m main image (ClassFacetMachine.
+setFacetsLocs): facet centres ``pixCentral``, facet (l0, m0), and the spatial weights ``SW`` --
+the facet polygon mask convolved with a Gaussian of ``MixingWidth`` pixels, normalised to 1 and
+clipped below 1e-3, built the way ClassFacetMachine._initcf_worker builds it
+(ClassFacetMachine.py:785-818, ModFFTW.py:654-689, 760-847).  Main-image pixel (X, Y) maps to (l, m) = ((X - m//2) cell, (Y - m//2) cell), the
 orientation fixed by ClassDDEGridMachine.testGrid (ClassDDEGridMachine.py:73-83, 229-271) and
 the facet flips of FacetsToIm_Channel / GiveModelTessel.
 
@@ -22,15 +22,21 @@ from .device import stream_handle
 
 
 class FacetGeometry:
+    """The regular nfx x nfx tessellation of ClassFacetMachine.setFacetsLocs
+    (ClassFacetMachine.py:414-485): the m x m main image is nfx facets of facet_npix pixels a
+    side (m = nfx * facet_npix), facet centres one facet apart, each facet a square polygon of
+    half-width facet_npix / 2 pixels around its centre, gridded on a padded_npix grid."""
+
     def __init__(self, m, nfx, facet_npix, padded_npix, cell_rad, mixing_width=10.0):
         self.m = int(m)
         self.nfx = int(nfx)
         self.nf = self.nfx * self.nfx
         self.facet_npix = int(facet_npix)
+        if self.m != self.nfx * self.facet_npix:
+            raise ValueError("the main image must be nfx facets wide")
         self.n = int(padded_npix)
         self.cell = float(cell_rad)
-        step = self.m / self.nfx
-        centres = [int(round(step * (i + 0.5))) for i in range(self.nfx)]
+        centres = [i * self.facet_npix + self.facet_npix // 2 for i in range(self.nfx)]
         self.pix = [(centres[ix], centres[iy]) for iy in range(self.nfx) for ix in range(self.nfx)]
         c = self.m // 2
         self.lm = [((x - c) * self.cell, (y - c) * self.cell) for x, y in self.pix]
@@ -38,22 +44,41 @@ class FacetGeometry:
         self._sw = None
 
     def spatial_weights(self):
-        """SW of every facet, float32 [nf, n, n] (image orientation, cf_dict["SW"])."""
+        """SW of every facet, float32 [nf, n, n], axes (l, m) (cf_dict["SW"],
+        ClassFacetMachine._initcf_worker, ClassFacetMachine.py:785-818):
+
+        * the facet polygon (and the image's outer square) tested at the points of
+          mgrid[l0 - R : l0 + R : n j] with R = n cell / 2, i.e. n points spaced n / (n - 1)
+          pixels apart (strict containment; the regular polygons' edges fall between samples);
+        * convolved with the unit-peak Gaussian of ModFFTW.GiveGauss(n, 1, (w, w, 0)), sampled on
+          the same n / (n - 1) spacing, as ModFFTW._convolveSingleGaussianFFTW does it: both
+          zero-padded to EstimateNpix(n) (no padding when n is already a fast size of at least
+          513), multiplied in the Fourier domain by |rfft2(PSF)|, i.e. a circular convolution;
+        * divided by its maximum, values below 1e-3 set to 0.
+        """
         if self._sw is not None:
             return self._sw
-        from scipy.ndimage import gaussian_filter
+        from .configs import estimate_npix
         n, m = self.n, self.m
+        s = n / (n - 1.0)
+        off = (np.arange(n) - (n - 1) / 2.0) * s              # sample offsets from the facet centre (px)
+        half = self.facet_npix / 2.0
+        u = -n / 2.0 + np.arange(n) * s                       # GiveGauss grid, uvscale = n / 2
+        g1 = np.exp(-u * u / (2.0 * self.mixing_width ** 2))
+        npad = max(n, estimate_npix(n)[1])
+        pe = max(int(np.ceil((estimate_npix(n)[1] - n) / 2.0) * 2), 0) // 2
+        psf = np.zeros((npad, npad))
+        psf[pe:pe + n, pe:pe + n] = np.outer(g1, g1)
+        fpsf = np.abs(np.fft.rfft2(np.fft.ifftshift(psf)))
         out = np.zeros((self.nf, n, n), np.float32)
-        half = self.m / self.nfx / 2.0
-        a = np.arange(n)
         for i, (xc, yc) in enumerate(self.pix):
-            X = xc - n // 2 + a
-            Y = yc - n // 2 + a
-            inx = (np.abs(X - xc) <= half) & (X >= 0) & (X < m)
-            iny = (np.abs(Y - yc) <= half) & (Y >= 0) & (Y < m)
-            mask = (inx[:, None] & iny[None, :]).astype(np.float64)
-            sw = gaussian_filter(mask, self.mixing_width, mode="constant")
-            sw /= sw.max()
+            inx = (np.abs(off) < half) & (np.abs(xc - m // 2 + off) < m / 2.0)
+            iny = (np.abs(off) < half) & (np.abs(yc - m // 2 + off) < m / 2.0)
+            mask = np.zeros((npad, npad))
+            mask[pe:pe + n, pe:pe + n] = np.outer(inx, iny)
+            a = np.fft.fftshift(np.fft.irfft2(np.fft.rfft2(np.fft.ifftshift(mask)) * fpsf, s=mask.shape))
+            sw = a[pe:pe + n, pe:pe + n]
+            sw = sw / sw.max()
             sw[sw < 1e-3] = 0.0
             out[i] = sw
         self._sw = out

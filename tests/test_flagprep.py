@@ -1,7 +1,7 @@
 """Flag preparation (ClassMS.py:1155-1167) against hand-worked cases."""
 import numpy as np
 
-from ddfacet_amd.flagprep import prepare_flags
+from oracle.flagprep import prepare_flags
 
 
 def test_autocorrelations_nan_and_equalisation():

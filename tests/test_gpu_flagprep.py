@@ -1,5 +1,5 @@
 """The chunk flag preparation on the device (flagprep.prepare_flags_device, ddf_prepare_flags)
-against the host restatement (flagprep.prepare_flags, pinned in tests/test_flagprep.py): flags and
+against the host restatement (oracle/flagprep.py, pinned in tests/test_flagprep.py): flags and
 data identical, bit for bit, with autocorrelation rows, NaN visibilities (real or imaginary part)
 and partially flagged correlations, for 4 and 2 correlations and without data."""
 import numpy as np
@@ -24,7 +24,8 @@ def _chunk(seed, nrow=700, nchan=9, ncorr=4):
 
 @pytest.mark.parametrize("ncorr,with_data", [(4, True), (2, True), (4, False)])
 def test_prepare_flags_device_matches_host(ncorr, with_data):
-    from ddfacet_amd.flagprep import prepare_flags, prepare_flags_device
+    from ddfacet_amd.flagprep import prepare_flags_device
+    from oracle.flagprep import prepare_flags
     print("case", ncorr, with_data, flush=True)
     flags, data, a0, a1 = _chunk(3 + ncorr, ncorr=ncorr)
     dev = torch.device("cuda")
@@ -42,7 +43,8 @@ def test_prepare_flags_device_matches_host(ncorr, with_data):
 def test_prepare_flags_device_on_unaligned_views():
     """Flags and values that start one (row, channel) into a buffer (values 8-byte but not 16-byte
     aligned): the word / 16-byte path is not taken, the result is the same."""
-    from ddfacet_amd.flagprep import prepare_flags, prepare_flags_device
+    from ddfacet_amd.flagprep import prepare_flags_device
+    from oracle.flagprep import prepare_flags
     flags, data, a0, a1 = _chunk(11, nrow=300, nchan=5, ncorr=4)
     dev = torch.device("cuda")
     bf = torch.zeros(flags.size + 4, dtype=torch.uint8, device=dev)

@@ -106,3 +106,48 @@ def test_estimate_npix_matches_workload_sizes():
     assert estimate_npix(4096 / 9, 1.7) == (455, 735)
     assert estimate_npix(6144 / 11, 1.7)[1] == 975
     assert estimate_npix(2048 / 5, 1.7)[1] == 693
+
+
+def _box_gauss_profile(n, half, sigma):
+    """One axis of SW for a facet clear of the image edge, by direct summation: the polygon test
+    on samples spaced s = n / (n - 1) pixels keeps the 2J + 1 central samples with J s < half;
+    each output sample sums the Gaussian exp(-(d s)^2 / (2 sigma^2)) over them (d in samples).
+    Near the box edge it follows the normal-CDF step of width sigma / s samples."""
+    s = n / (n - 1.0)
+    c = n // 2
+    J = int(np.floor(half / s - 1e-12))
+    x = np.arange(n) - c
+    j = np.arange(-J, J + 1)
+    f = np.exp(-((x[:, None] - j[None, :]) * s) ** 2 / (2 * sigma ** 2)).sum(1)
+    return f / f[c]
+
+
+@pytest.mark.parametrize("m,nfx,n,sigma", [(243, 3, 147, 4.0), (405, 5, 147, 2.5), (2025, 5, 693, 10.0)])
+def test_spatial_weights_closed_form(m, nfx, n, sigma):
+    """SW of the central facet (no image edge within reach) equals the separable box-Gaussian
+    profile of the reference construction summed directly (no FFT, no padding); the step across
+    the polygon edge has the normal-CDF width sigma / s; an image-corner facet is cut at the image
+    edge."""
+    from scipy.special import ndtr
+    from ddfacet_amd.imaging import FacetGeometry
+    g = FacetGeometry(m, nfx, m // nfx, n, 1e-5, mixing_width=sigma)
+    sw = g.spatial_weights()
+    fi = g.nf // 2
+    f = _box_gauss_profile(n, (m // nfx) / 2.0, sigma)
+    want = np.outer(f, f)
+    want[want < 1e-3] = 0
+    assert np.abs(sw[fi] - want).max() <= 2e-6
+    c = n // 2
+    assert sw[fi][c, c] == 1.0
+    J0 = int(np.floor((m // nfx) / 2.0 * (n - 1) / n))
+    x = np.arange(c + J0 - 3, c + J0 + 5) - c
+    erf_step = ndtr((J0 + 0.5 - x) / (sigma * (n - 1) / n))
+    np.testing.assert_allclose(sw[fi][c, c + x], erf_step, atol=3e-3)
+    # edge samples of a regular facet sit symmetrically about its polygon edge
+    J = int(np.floor((m // nfx) / 2.0 * (n - 1) / n))
+    np.testing.assert_allclose(sw[fi][c, c + J] + sw[fi][c, c + J + 1], 1.0, atol=1e-3)
+    # corner facet 0: samples outside the main image are masked before the convolution
+    x0 = g.pix[0][0]
+    outside = (np.arange(n) - c) * n / (n - 1.0) + x0 - m // 2 <= -m / 2.0
+    assert outside.any()
+    assert sw[0][c, outside].max() < 0.5 and sw[0][c, c] > 0.99
