@@ -9,7 +9,11 @@ visibilities presented, flagged ones included).
 
 Roofline (HBM-bound scatter/gather).  `roofline` is for the dominant pass (the batched pass with
 the largest mean duration, timed live with HIP events on its launch stream through
-ddf_plan_timing): achieved = that pass's algorithmic bytes / its mean duration, with
+ddf_plan_timing): achieved = the pass's share of the per-facet algorithm's bytes (per
+visibility-facet pair 40 + 64 + 48 / C: grid pass A carries 40 + 24 / C per pair, degrid pass B
+64 + 24 / C, the tile passes the per-facet grid and CF bytes) / its mean duration, frac =
+achieved / 8 TB/s (`passes_frac` for every pass).  `unique_frac` prices the same pass on the
+bytes it must move once (a visibility read once for all facets):
   grid pass A    per vis 40 (vis 32, flags 4, weight 4), per row 24 (uvw), per BDA word 4,
                  per record 8 (placement) + 8 per Stokes (block sum written), Jones tables once
   grid pass B    per record 12 (placement 8, index 4) + 8 per Stokes (block sum), per facet the CF
@@ -21,8 +25,11 @@ ddf_plan_timing): achieved = that pass's algorithmic bytes / its mean duration, 
 40 + 64 + 48 / C bytes, plus per facet 3 grids, the BDA maps and the CF table, over t_step; a
 design that reads a visibility once for all facets can exceed 1 there.
 The CPU baseline is the reference gridder (oracle/_ref, built from the reference sources) on the
-host cores (at most 16 worker processes, one facet each) over a stride sample of the same BDA
-blocks, scaled to the full cycle; it runs before the GPU is initialised.
+host cores (at most 16 worker processes, the CPU share of one GPU, over all facet jobs) on a
+stride sample of the same BDA blocks, scaled to the full cycle; it runs before the GPU is
+initialised.  Reported beside t_step, not in it: the plan build (`plan`: ms and device bytes,
+once per chunk), the image side (`image_side_ms`: facet FFTs, stitch, model cut) and the same
+cycle in the FP64 precision (`fp64_ms_per_step`, with `rel_err_vs_fp64`).
 """
 import argparse
 import json
@@ -96,6 +103,24 @@ def pass_bytes(P, nf, info, ns):
     }
 
 
+def survey_pass_bytes(P, nf):
+    """The per-facet algorithm's bytes (per visibility-facet pair 40 + 64 + 48 / C, per facet 3
+    grids and the CF table) split over the batched passes: grid pass A carries the gridding side
+    of every pair (40 + 24 / C), degrid pass B the degridding side (64 + 24 / C), the tile passes
+    the per-facet terms (grid B: grid zero + write and the CF table, degrid A: model read and the
+    CF table)."""
+    R, C = P.uvw.shape[0], P.freqs.shape[0]
+    pairs = R * C * nf
+    gbytes = P.nband * P.npol * P.npix * P.npix * 8
+    cf = _cf_bytes(P)
+    return {
+        "grid_pass_a": pairs * (40 + 24.0 / C),
+        "grid_pass_b": nf * (2 * gbytes + cf),
+        "degrid_pass_a": nf * (gbytes + cf),
+        "degrid_pass_b": pairs * (64 + 24.0 / C),
+    }
+
+
 def plain_bytes(P, kind):
     """Algorithmic bytes of one per-facet plain launch (grid_plain / degrid_plain)."""
     R, C = P.uvw.shape[0], P.freqs.shape[0]
@@ -120,12 +145,24 @@ def _cpu_facet_job(fi):
     return fi
 
 
-def cpu_baseline(P, target_s=8.0, max_workers=16):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(P, target_s=12.0, max_workers=16):
     """The reference gridder (oracle/_ref, built from DDFacet/Gridder) on the host cores, before
-    the GPU is touched: one facet per worker process (the reference's process pool runs one facet
-    job per core with one OpenMP thread, AsyncProcessPool.py:831-861), each worker gridding then
-    degridding a stride sample of the BDA blocks; 3373 row semaphores as the reference deploys.
-    value = vis-facet pairs per second over all workers / the number of facets."""
+    the GPU is touched: one facet job at a time per worker process (the reference's process pool
+    runs one facet job per core with one OpenMP thread, AsyncProcessPool.py:831-861), every facet
+    of the cycle gridded then degridded over the same stride sample of the BDA blocks; 3373 row
+    semaphores as the reference deploys.  Workers: the affinity cores, at most `max_workers` (the
+    CPU share of one GPU on the box).  value = vis-facet pairs per second over all workers / the
+    number of facets."""
     import multiprocessing as mp
     from ddfacet_amd import bda as bdamod
     from ddfacet_amd.problems import Problem
@@ -133,7 +170,8 @@ def cpu_baseline(P, target_s=8.0, max_workers=16):
     if not refmod.available():
         return None
     mod = refmod.module(nsem=3373)
-    cores = max(1, min(max_workers, len(os.sched_getaffinity(0))))
+    affinity = len(os.sched_getaffinity(0))
+    cores = max(1, min(max_workers, affinity))
     nb = bdamod.nblocks_of(P.bda_grid)
     starts = bdamod.block_starts(P.bda_grid)
     sizes = P.bda_grid[2:2 + nb]
@@ -168,34 +206,36 @@ def cpu_baseline(P, target_s=8.0, max_workers=16):
         return S, nvis_s
 
     nf = len(P.facets)
-    # calibrate the stride on one facet in this process, then one facet per core in parallel
+    # calibrate the stride on one facet in this process, then every facet over the workers
     _CPU.update(mod=mod)
     S, nv = sample(400)
     _CPU["S"] = S
     t0 = time.perf_counter()
     _cpu_facet_job(nf // 2)
     t1 = time.perf_counter() - t0
-    stride = int(max(1, min(400, round(400 * t1 / target_s))))
+    stride = int(max(1, min(4000, round(400 * nf * t1 / (cores * target_s)))))
     S, nv = sample(stride)
     _CPU["S"] = S
-    facets = sorted(set(int(x) for x in np.linspace(0, nf - 1, min(cores, nf)).round()))
+    facets = list(range(nf))
     ctx = mp.get_context("fork")
-    with ctx.Pool(len(facets)) as pool:
+    with ctx.Pool(cores) as pool:
         t0 = time.perf_counter()
         pool.map(_cpu_facet_job, facets, chunksize=1)
         t = time.perf_counter() - t0
     _CPU.clear()
-    vf_per_s = nv * len(facets) / t
-    return dict(value=vf_per_s / nf, unit="vis/s", cores=len(facets), kind="reference",
+    vf_per_s = nv * nf / t
+    return dict(value=vf_per_s / nf, unit="vis/s", cores=cores, kind="reference", cpu_model=cpu_model(),
+                affinity_cores=affinity,
                 sample="reference _pyGridderSmearPols3x (oracle/_ref, built from DDFacet/Gridder sources), "
-                       "%d worker processes (one facet each, facets %s), grid + degrid of every %d-th BDA "
-                       "block (%d vis), %.1f s wall; value = vis-facet pairs/s / %d facets"
-                       % (len(facets), facets, stride, nv, t, nf))
+                       "%d worker processes (of %d affinity cores; one GPU's CPU share is 16) running all %d "
+                       "facet jobs, each gridding then degridding every %d-th BDA block (%d vis), %.1f s wall; "
+                       "value = vis-facet pairs/s / %d facets" % (cores, affinity, nf, stride, nv, t, nf))
 
 
-def error_vs_fp64(fs, vis0, tile, group, blocks_per_item, stream):
+def error_vs_fp64(fs, vis0, tile, group, blocks_per_item, stream, steps=3):
     """Relative L2 distance of the timed (mixed-precision) cycle's facet grids and predicted
-    visibilities to the same cycle run by an FP64-precision plan of the same chunk (untimed)."""
+    visibilities to the same cycle run by an FP64-precision plan of the same chunk, and that
+    plan's cycle time (mean of `steps` cycles after one warm-up)."""
     import torch
 
     def cycle():
@@ -218,9 +258,82 @@ def error_vs_fp64(fs, vis0, tile, group, blocks_per_item, stream):
     g_mixed, p_mixed = cycle()
     fs.plan_batched(tile=tile, group=group, blocks_per_item=blocks_per_item, precision="fp64", stream=stream)
     g64, p64 = cycle()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fs.zero_grids()
+        fs.grid_batched(stream=stream)
+        fs.degrid_batched(stream=stream)
+    torch.cuda.synchronize()
+    ms64 = (time.perf_counter() - t0) * 1e3 / steps
+    fs.t["vis"].copy_(vis0)
     fs.destroy_plan()
-    return {"grids": rel(g_mixed, g64), "predicted_vis": rel(p_mixed, p64),
+    return {"grids": rel(g_mixed, g64), "predicted_vis": rel(p_mixed, p64), "fp64_ms_per_step": ms64,
             "note": "the timed cycle's arithmetic (mixed precision) against the FP64-precision plan"}
+
+
+def device_bda_map(P, steps):
+    """Time the chunk's BDA map built on the device (ddf_bda_build_*, the mode-1 builder with the
+    workload's GridDecorr 0.02 and facet field of view), inputs resident in HBM.  Returns the
+    timing and the map (host copy)."""
+    import torch
+    from ddfacet_amd import bda as bdamod
+    cell = P.extra["cell_arcsec"] * np.pi / 180 / 3600
+    dfreq = float(P.freqs[1] - P.freqs[0])
+    t_uvw = torch.from_numpy(np.ascontiguousarray(P.uvw)).cuda()
+    t_a0 = torch.from_numpy(np.int32(P.a0)).cuda()
+    t_a1 = torch.from_numpy(np.int32(P.a1)).cuda()
+    t_cm = torch.from_numpy(np.int32(P.chan_map)).cuda()
+    args = (t_uvw, t_a0, t_a1, P.freqs, dfreq, 0.02, cell, P.extra["facet_npix"], t_cm)
+    m = bdamod.bda_map_device(*args)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(max(1, steps)):
+        t0 = time.perf_counter()
+        bdamod.bda_map_device(*args)
+        torch.cuda.synchronize()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    return {"ms": float(np.median(ts)), "blocks": bdamod.nblocks_of(m.cpu().numpy()[:2]), "words": int(m.numel()),
+            "map": m.cpu().numpy()}
+
+
+def image_side_ms(P, fs, steps):
+    """Mean ms of the image side of the cycle, reported apart from t_step (ClassFacetMachine.py:
+    1331-1416 stitch, ClassDDEGridMachine.py:1186-1190 GridToIm, ClassImToGrid.py:138-264 +
+    get()'s FFT): GridToIm of every facet grid (batched rocFFT + shifts, x OverS^2), the stitch
+    onto the 4095^2 main image, and the model cut + forward FFT of every facet."""
+    import torch
+    from ddfacet_amd import imaging
+    from ddfacet_amd.imaging import FacetGeometry
+    nfx = P.extra["nfx"]
+    fn = P.extra["facet_npix"]
+    geom = FacetGeometry(nfx * fn, nfx, fn, P.npix, P.extra["cell_arcsec"] * np.pi / 180 / 3600)
+    dev = fs.grids.device
+    sphe = torch.from_numpy(np.stack([np.float32(f.sphe) for f in P.facets])).to(dev)
+    inv = torch.from_numpy(np.stack([np.float32(1.0 / np.float64(f.sphe)) for f in P.facets])).to(dev)
+    sw = torch.from_numpy(geom.spatial_weights()).to(dev)
+    norm = torch.zeros((geom.m, geom.m), dtype=torch.float32, device=dev)
+    imaging.facet_norm(geom, sw, norm)
+    image = torch.zeros((P.nband, P.npol, geom.m, geom.m), dtype=torch.float32, device=dev)
+    overs = int(P.facets[0].winfos[3])
+    sj = fs.sum_jones if fs.jones is not None else None
+    parts = {"grid_to_image": lambda: imaging.grid_to_image(fs.grids, overs),
+             "stitch": lambda: imaging.stitch(geom, sphe, inv, sw, fs.grids, fs.sumwt, sj, norm, image),
+             "model_to_grid": lambda: imaging.model_to_grid(geom, sphe, sw, image, norm, fs.models, overs)}
+    keep = fs.models.clone()
+    out = {}
+    for k, fn_ in parts.items():
+        fn_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn_()
+        torch.cuda.synchronize()
+        out[k] = (time.perf_counter() - t0) * 1e3 / steps
+    out["total"] = sum(out.values())
+    fs.models.copy_(keep)
+    fs.zero_grids()
+    return out
 
 
 def time_graph(fs, steps):
@@ -294,6 +407,132 @@ def run_uneven(a):
         "plain_ms_per_step": ms_plain, "plain_value": R * C / (ms_plain * 1e-3),
         "speedup_vs_plain": ms_plain / ms_batched, "graph_ms_per_step": ms_graph,
         "autotune": {"best": best, "candidates": [[c, ms] for c, ms in results]},
+    }
+    print(json.dumps(out))
+
+
+def run_single(a):
+    """The plain single-facet workload (561 600 visibilities, one 512^2 grid): the reference
+    gridder (oracle/_ref) and the oracle on one host core, the plain per-facet kernels behind the
+    drop-in shim (inputs resident in HBM) and the batched plan, each timed over grid + degrid."""
+    import torch
+    from ddfacet_amd import configs
+    from ddfacet_amd.facets import DeviceFacetSet
+    from oracle import cpuref, refmod
+    P = configs.cfg1(log=log)
+    R, C = P.uvw.shape[0], P.freqs.shape[0]
+    nvis = R * C
+    rng = np.random.default_rng(8)
+    model = (rng.standard_normal((1, 1, P.npix, P.npix)) * 1e-3).astype(np.complex64)
+    cpu = {}
+    if not a.no_cpu:
+        impls = [("oracle", cpuref.grid, cpuref.degrid)]
+        if refmod.available():
+            m = refmod.module()
+            impls.insert(0, ("reference", m.pyGridderWPol32, m.pyDeGridderWPol))
+        for label, gf, df in impls:
+            ts = []
+            for _ in range(3):
+                g, sw = P.new_grid(), np.zeros((1, 1))
+                v = P.vis.copy()
+                t0 = time.perf_counter()
+                gf(*P.grid_args(0, g, sw))
+                df(*P.degrid_args(0, model, v))
+                ts.append(time.perf_counter() - t0)
+            cpu[label] = {"value": nvis / float(np.median(ts)), "unit": "vis/s", "cores": 1,
+                          "s_per_cycle": float(np.median(ts)), "cpu_model": cpu_model()}
+        log("cpu: %s" % cpu)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    fs = DeviceFacetSet(P, device=dev)
+    fs.models[0].copy_(torch.from_numpy(model[0]))
+    vis0 = fs.t["vis"].clone()
+
+    def timed(fn):
+        for _ in range(a.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            fn()
+        torch.cuda.synchronize()
+        fs.t["vis"].copy_(vis0)
+        return (time.perf_counter() - t0) * 1e3 / a.steps
+
+    def plain():
+        fs.zero_grids()
+        fs.grid_plain()
+        fs.degrid_plain()
+
+    def batched():
+        fs.zero_grids()
+        fs.grid_batched()
+        fs.degrid_batched()
+
+    ms_plain = timed(plain)
+    fs.plan_batched()
+    ms_batched = timed(batched)
+    out = {
+        "metric": "vis/sec grid+degrid, single 512^2 facet",
+        "value": nvis / (ms_batched * 1e-3), "unit": "vis/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms_batched, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 (fp32 grids, fp64 phase base)", "data": "synthetic",
+        "config": {"workload": "single 512^2 facet at 2\", 27 antennas in a 20 km disk, 100 x 10 s, 16 ch "
+                               "1.2-1.6 GHz, 8 w-planes, S = 7, OverS = 11, one BDA block per visibility",
+                   "rows": R, "channels": C, "nvis": nvis},
+        "plain_ms_per_step": ms_plain, "plain_value": nvis / (ms_plain * 1e-3),
+        "cpu_one_core": cpu,
+    }
+    fs.destroy_plan()
+    print(json.dumps(out))
+
+
+def run_bda(a):
+    """The 81-facet workload chunk's BDA map (mode 1, GridDecorr 0.02, facet field of view) built
+    on the device from uvw / antennas resident in HBM; CPU baseline: the host restatement of the
+    reference builder (oracle/bda.py) over the whole chunk on one core."""
+    import torch
+    from ddfacet_amd import bda as bdamod
+    from ddfacet_amd import configs, synth
+    freqs = np.linspace(0.9e9, 1.67e9, 64)
+    obs = synth.make_observation("meerkat", 64, 600, 8.0, freqs, -30.0, -30.7, 2000, sort_by_baseline=True)
+    cell = 1.5 * synth.ARCSEC
+    nopt, _ = configs.estimate_npix(4096 / 9, 1.7)
+    chan_map = np.zeros(64, np.int32)
+    R = obs.nrow
+    cpu, host = None, None
+    if not a.no_cpu:
+        from oracle import bda as obda
+        t0 = time.perf_counter()
+        host = obda.bda_map_mode1(obs.uvw, obs.a0, obs.a1, freqs, obs.dfreq, 0.02, cell, nopt, chan_map)
+        dt = time.perf_counter() - t0
+        cpu = {"value": R / dt, "unit": "rows/s", "cores": 1, "kind": "port",
+               "sample": "host restatement of GiveBlocksRowsListBL_old + collectSmearMapping (oracle/bda.py) on "
+                         "the whole chunk (%d rows), %.1f s" % (R, dt)}
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    args = (torch.from_numpy(np.ascontiguousarray(obs.uvw)).cuda(), torch.from_numpy(np.int32(obs.a0)).cuda(),
+            torch.from_numpy(np.int32(obs.a1)).cuda(), freqs, obs.dfreq, 0.02, cell, nopt,
+            torch.from_numpy(chan_map).cuda())
+    for _ in range(a.warmup):
+        m = bdamod.bda_map_device(*args)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        m = bdamod.bda_map_device(*args)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    mh = m.cpu().numpy()
+    out = {
+        "metric": "rows/sec BDA block map build, 81-facet MeerKAT-like chunk",
+        "value": R / (ms * 1e-3), "unit": "rows/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 builder arithmetic, int32 map", "data": "synthetic",
+        "config": {"workload": "BDA map (mode 1, GridDecorr 0.02, facet FoV) of the 81-facet MeerKAT-like chunk: "
+                               "64 antennas, 600 x 8 s, 64 ch", "rows": R, "blocks": bdamod.nblocks_of(mh),
+                   "words": int(mh.size)},
+        "equal_to_host_builder": None if host is None else bool(np.array_equal(mh, host)),
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
@@ -484,13 +723,16 @@ def main():
     ap.add_argument("--mode", default="batched", choices=["batched", "plain"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--facets", type=int, default=0, help="use only the first N facets (debug)")
-    ap.add_argument("--workload", default="meerkat81", choices=["meerkat81", "uneven121", "autograd25", "weights", "flags"],
-                    help="meerkat81: the 81-facet MeerKAT-like cycle (the headline line); uneven121: "
+    ap.add_argument("--workload", default="meerkat81",
+                    choices=["meerkat81", "single512", "uneven121", "autograd25", "weights", "flags", "bda"],
+                    help="meerkat81: the 81-facet MeerKAT-like cycle (the headline line); single512: the "
+                         "plain single-facet workload, CPU and GPU plain / batched; uneven121: "
                          "121 facets x 4 bands with baseline-dependent averaging, plain per-facet "
                          "launches against the autotuned batched cycle (eager and graph-replayed); "
                          "autograd25: 20 chi^2 gradient-descent steps through the differentiable "
                          "degridder on 25 facets x 4 bands; weights: Briggs imaging weights of the "
-                         "81-facet workload's chunk; flags: flag preparation of that chunk")
+                         "81-facet workload's chunk; flags: flag preparation of that chunk; bda: the "
+                         "chunk's BDA block map built on the device")
     ap.add_argument("--tile", type=int, default=0, help="uv tile edge of the batched plan (0: the plan default)")
     ap.add_argument("--blocks-per-item", type=int, default=0,
                     help="BDA blocks per pass-A work item of the batched plan (0: the plan default)")
@@ -499,7 +741,9 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="also time the batched cycle replayed from a HIP graph")
     ap.add_argument("--no-fp64-check", action="store_true",
-                    help="skip the untimed FP64-precision cycle that measures the error of the timed one")
+                    help="skip the FP64-precision cycle that measures the error (and the time) against the timed one")
+    ap.add_argument("--no-image", action="store_true",
+                    help="skip the separately reported image side (facet FFTs, stitch, model cut)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only an elementwise copy of 1 GiB (reads 1 GiB, writes 1 GiB), to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiler run")
@@ -522,16 +766,26 @@ def main():
 
     if a.workload == "uneven121":
         return run_uneven(a)
+    if a.workload == "single512":
+        return run_single(a)
     if a.workload == "autograd25":
         return run_autograd(a)
     if a.workload == "weights":
         return run_weights(a)
     if a.workload == "flags":
         return run_flags(a)
+    if a.workload == "bda":
+        return run_bda(a)
     from ddfacet_amd import configs
     t0 = time.time()
-    # geometry, maps, CFs and Jones on the host; the CPU baseline runs before the GPU is touched
-    P = configs.cfg2(with_vis=False, log=log)
+    # geometry, maps, CFs and Jones; the CPU baseline runs before the GPU is touched, so with it
+    # the BDA map comes from the host restatement of the builder (the same map word for word as
+    # the device builder's, which is then checked and timed once the GPU is up)
+    host_bda = None
+    if not a.no_cpu:
+        from oracle import bda as obda
+        host_bda = obda.bda_map_mode1
+    P = configs.cfg2(with_vis=False, log=log, bda_builder=host_bda)
     if a.facets:
         P.facets = P.facets[:a.facets]
     cpu = None
@@ -546,6 +800,12 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     P.vis = configs.cfg2_visibilities(P.uvw, P.freqs, P.extra["sky"], device="cuda")
+    bda_dev = device_bda_map(P, a.steps)
+    if host_bda is not None:
+        bda_dev["equal_to_host_builder"] = bool(np.array_equal(bda_dev.pop("map"), P.bda_grid))
+    else:
+        bda_dev.pop("map")
+    log("device BDA map: %s" % bda_dev)
     fs = DeviceFacetSet(P, device=dev)
     nf = fs.nf
     gen = torch.Generator(device=dev).manual_seed(7)
@@ -554,11 +814,18 @@ def main():
     log("setup %.1fs" % (time.time() - t0))
 
     stream = torch.cuda.current_stream()
+    plan = None
     if a.mode == "batched":
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
         fs.plan_batched(tile=a.tile, group=a.group, blocks_per_item=a.blocks_per_item, stream=stream)
+        torch.cuda.synchronize()
+        plan_ms = (time.perf_counter() - tp) * 1e3
         fs.plan_timing(True)
         info = fs.plan_info()
-        log("plan: %s" % info)
+        plan = {"ms": plan_ms, "device_bytes": info["device_bytes"],
+                "note": "placements, bins and stamp records of both maps; once per chunk, untimed in the cycle"}
+        log("plan: %s, built in %.1f ms" % (info, plan_ms))
     ev = {"grid": [], "degrid": []}
     passes = []
 
@@ -606,10 +873,14 @@ def main():
         ns = len(P.stokes_codes)
         pms = {k: float(np.mean([d[k] for d in passes])) for k in passes[0] if passes[0][k] is not None}
         pb = pass_bytes(P, nf, info, ns)
+        sb = survey_pass_bytes(P, nf)
         dom = max(pms, key=pms.get)
-        launch_ms, abytes = pms[dom], pb[dom]
+        launch_ms, abytes = pms[dom], sb[dom]
         extra = {"passes_ms": pms,
-                 "passes_gbps": {k: pb[k] / (pms[k] * 1e-3) / 1e9 for k in pms if k in pb}}
+                 "passes_frac": {k: sb[k] / (pms[k] * 1e-3) / HBM_PEAK for k in pms if k in sb},
+                 "unique_bytes_per_launch": pb[dom],
+                 "unique_frac": pb[dom] / (launch_ms * 1e-3) / HBM_PEAK,
+                 "passes_unique_gbps": {k: pb[k] / (pms[k] * 1e-3) / 1e9 for k in pms if k in pb}}
     else:
         dom = "grid_plain" if tg >= td else "degrid_plain"
         launch_ms = (tg if dom == "grid_plain" else td) / nf
@@ -623,8 +894,12 @@ def main():
         graph_ms = time_graph(fs, a.steps)
         fs.t["vis"].copy_(vis0)
     err64 = None
+    image = None
+    if a.mode == "batched" and not a.no_image:
+        image = image_side_ms(P, fs, a.steps)
+        log("image side: %s" % (image,))
     if a.mode == "batched" and not a.no_fp64_check:
-        err64 = error_vs_fp64(fs, vis0, a.tile, a.group, a.blocks_per_item, stream)
+        err64 = error_vs_fp64(fs, vis0, a.tile, a.group, a.blocks_per_item, stream, a.steps)
     cfg = {"workload": "81-facet MeerKAT-like: 9x9 facets (455 -> 735 px at 1.5\"), 64 antennas, "
                        "600 x 8 s, 64 ch 0.9-1.67 GHz, 4 corr, per-visibility DD Jones, default BDA (0.02)",
            "rows": R, "channels": C, "nvis": nvis, "facets": nf,
@@ -647,7 +922,13 @@ def main():
     if graph_ms is not None:
         out["graph_ms_per_step"] = graph_ms
     if err64 is not None:
+        out["fp64_ms_per_step"] = err64.pop("fp64_ms_per_step")
         out["rel_err_vs_fp64"] = err64
+    if plan is not None:
+        out["plan"] = plan
+    if image is not None:
+        out["image_side_ms"] = image
+    out["bda_map"] = bda_dev
     print(json.dumps(out))
 
 

@@ -171,6 +171,13 @@ class ddf_cf_spec(C.Structure):
 
 PRECISION = {"mixed": 0, "fp64": 1, "fp32": 2}
 
+
+class ddf_bda_spec(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("nchan", C.c_int32), ("na", C.c_int32), ("per_row", C.c_int32),
+                ("duv_cut", C.c_double), ("k_dnu", C.c_double), ("l", C.c_double), ("dfreq", C.c_double),
+                ("k_phase", C.c_double), ("lmax", C.c_double), ("nmax", C.c_double), ("k_span", C.c_double),
+                ("dphi", C.c_double)]
+
 EXTRA_SIGNATURES = {
     "ddf_weights_index": [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_int64,
                           C.c_int64, C.c_int, C.c_int64, C.c_void_p, C.c_void_p],
@@ -185,6 +192,10 @@ EXTRA_SIGNATURES = {
     "ddf_plan_degrid": [C.c_void_p, C.c_void_p, C.c_void_p],
     "ddf_plan_info": [C.c_void_p, C.POINTER(C.c_int64), C.c_int32],
     "ddf_plan_placements": [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p],
+    "ddf_bda_build_create": [C.POINTER(ddf_bda_spec), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                             C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int64), C.c_void_p],
+    "ddf_bda_build_write": [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p],
+    "ddf_bda_build_destroy": [C.c_void_p],
     "ddf_cf_build": [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p],
     "ddf_plan_timing": [C.c_void_p, C.c_int32],
     "ddf_plan_pass_ms": [C.c_void_p, C.POINTER(C.c_float), C.c_int32],

@@ -6,12 +6,13 @@ with rec_b = [chStart, chEnd, row, row, ...] and size_b = len(rec_b) = 2 + rows_
 block lists are concatenated in lexicographic order of the string key "a0:a1"
 (ClassSmearMapping.py:113).
 
-``bda_map_mode1`` restates the default block builder ``GiveBlocksRowsListBL_old``
-(ClassSmearMapping.py:496-625, ``--Comp-BDAMode=1``) and ``bda_map_mode2`` the phase-based builder
-``GiveBlocksRowsListBL`` (ClassSmearMapping.py:348-491, ``--Comp-BDAMode=2``), both driven as
-``computeSmearMappingInBackground`` drives them (ClassSmearMapping.py:64-79,
-ClassVisServer.py:637-679).  The other builders are synthetic layouts used by the benchmarks: one
-block per (row, channel) and fixed time x frequency averaging factors per baseline.
+``bda_map_device`` builds the maps of the default block builder ``GiveBlocksRowsListBL_old``
+(ClassSmearMapping.py:496-625, ``--Comp-BDAMode=1``) and of the phase-based builder
+``GiveBlocksRowsListBL`` (ClassSmearMapping.py:348-491, ``--Comp-BDAMode=2``) on the GPU
+(ddf_bda_build_*), in place of ``computeSmearMappingInBackground`` / ``collectSmearMapping``
+(ClassSmearMapping.py:64-135, ClassVisServer.py:637-679).  The other builders are synthetic
+layouts used by the benchmarks: one block per (row, channel) and fixed time x frequency
+averaging factors per baseline.
 """
 import numpy as np
 
@@ -76,142 +77,6 @@ def _baseline_rows(a0, a1):
     return out
 
 
-def _blocks_bl_old(ind, uvw, dfreq, freqs, dphi, lrad, chan_map):
-    """One baseline of GiveBlocksRowsListBL_old (ClassSmearMapping.py:496-625)."""
-    nrows = ind.size
-    nchan = freqs.size
-    nu0 = np.max(freqs)
-    u, v, w = uvw[ind, :].T
-    du, dv, dw = u[1:] - u[:-1], v[1:] - v[:-1], w[1:] - w[:-1]
-    if du.size < 1:
-        du, dv, dw = np.array([1e-30]), np.array([1e-30]), np.array([1e-30])
-    du = np.concatenate((du, [du[-1]]))
-    dv = np.concatenate((dv, [dv[-1]]))
-    dw = np.concatenate((dw, [dw[-1]]))
-    duv_cut = C_BDA * dphi / (np.pi * lrad * nu0)
-    uvd = np.maximum(np.sqrt(u ** 2 + v ** 2 + w ** 2), 1e-10)
-    nch_block = ((C_BDA / np.pi) * dphi / (uvd * lrad)) / dfreq
-    step = np.sqrt(du ** 2 + dv ** 2 + dw ** 2)
-    sizes, flat = [], []
-    cur = []
-    duvtot = 0.0
-    nchb_max = 1e3
-    for i in range(nrows):
-        cur.append(int(ind[i]))
-        if nch_block[i] < nchb_max:
-            nchb_max = nch_block[i]
-        duvtot += step[i]
-        if (duvtot > duv_cut) or (i == nrows - 1):
-            nchb_max = np.max([nchb_max, 1])
-            ch = np.arange(0, nchan, nchb_max).tolist()
-            if nchan not in ch:
-                ch.append(nchan)
-            chb = np.int32(np.linspace(0, nchan, len(ch)))
-            rampe = np.zeros(nchan, np.int32)
-            for k in range(chb.size - 1):
-                rampe[chb[k]:chb[k + 1]] = k
-            cuts = []
-            prev = (-1, -1)
-            for c in range(nchan):
-                cur_key = (rampe[c], chan_map[c])
-                if cur_key != prev:
-                    cuts.append(c)
-                prev = cur_key
-            if chb[-1] not in cuts:
-                cuts.append(chb[-1])
-            cuts = np.array(cuts, np.int32)
-            for k in range(cuts.size - 1):
-                rec = [int(cuts[k]), int(cuts[k + 1])] + cur
-                sizes.append(len(rec))
-                flat.extend(rec)
-            nchb_max = 1e3
-            cur = []
-            duvtot = 0.0
-    return sizes, flat
-
-
-def bda_map_mode1(uvw, a0, a1, freqs, dfreq, decorr, cell_rad, facet_npix, chan_map):
-    """BDA.Grid / BDA.Degrid map of the default builder.
-
-    decorr: --Comp-GridDecorr (or DegridDecorr); the builder receives (1 - decorr) and uses
-    dPhi = sqrt(6 (1 - (1 - decorr))) (ClassSmearMapping.py:66, ClassVisServer.py:664-666).
-    cell_rad, facet_npix: the FoV radius is cell * npix * sqrt(2)/2 (ClassVisServer.py:661-663,
-    --Comp-GridFoV=Facet uses the non-padded facet size).
-    """
-    decorr_in = 1.0 - decorr
-    dphi = np.sqrt(6. * (1. - decorr_in))
-    fov_deg = cell_rad * facet_npix * (np.sqrt(2.) / 2.) * 180. / np.pi
-    lrad = fov_deg * np.pi / 180
-    chan_map = np.asarray(chan_map)
-    per_bl = {}
-    for key, ind in _baseline_rows(np.asarray(a0), np.asarray(a1)).items():
-        per_bl[key] = _blocks_bl_old(ind, uvw, dfreq, np.asarray(freqs), dphi, lrad, chan_map)
-    return pack(per_bl)
-
-
-def _blocks_bl_mode2(ind, uvw, dfreq, freqs, dphi, lmax, chan_map):
-    """One baseline of GiveBlocksRowsListBL (ClassSmearMapping.py:348-491).
-
-    Rows: the phase change to the next row at the facet edge, 2 pi nu_max / C (|du,dv| lmax +
-    |dw| nmax) with nmax = |sqrt(1 - lmax^2) - 1| (the last row repeats the previous step), is
-    accumulated; row i belongs to time block floor(cumsum_i / dPhi), and a block is cut where that
-    number changes.  Channels: per row the largest channel span of phase change dPhi,
-    (C / 2 pi) dPhi / (|u,v| lmax + |w| nmax) / dfreq; per time block its minimum (at least 1)
-    gives n = ceil(nchan / min) channel blocks of size ceil(nchan / n); channel c is in block
-    c // size, and blocks are also cut where chan_map changes.
-    Differences: a baseline with a single row has no step to repeat (the reference indexes the
-    row before it); its step is taken as 0.  A block whose channel span is unbounded (all rows at
-    uvw = 0) gets one channel block.
-    """
-    nrows = ind.size
-    nchan = freqs.size
-    nu0 = float(np.max(freqs))
-    x = uvw[ind, :]
-    d = np.zeros_like(x)
-    if nrows > 1:
-        d[:-1] = x[:-1] - x[1:]
-        d[-1] = d[-2]
-    nmax = abs(np.sqrt(1.0 - lmax ** 2) - 1.0)
-    k = 2.0 * np.pi * nu0 / C_BDA
-    dph = k * (np.sqrt(d[:, 0] ** 2 + d[:, 1] ** 2) * lmax + np.abs(d[:, 2]) * nmax)
-    ext = np.sqrt(x[:, 0] ** 2 + x[:, 1] ** 2) * lmax + np.abs(x[:, 2]) * nmax
-    with np.errstate(divide="ignore"):
-        span = (C_BDA / (2.0 * np.pi)) * dphi / ext / dfreq
-    if C_BDA * dphi / (2.0 * np.pi * nu0):
-        blk = np.int32(np.cumsum(dph) / dphi)
-        cut = [0] + [i for i in range(1, nrows) if blk[i] != blk[i - 1]] + [nrows]
-    else:
-        cut = list(range(nrows + 1))
-    chan_map = np.asarray(chan_map)
-    sizes, flat = [], []
-    for t in range(len(cut) - 1):
-        r0, r1 = cut[t], cut[t + 1]
-        m = max(float(np.min(span[r0:r1])), 1.0)
-        nblk = np.ceil(nchan / m) if np.isfinite(m) else 1.0
-        size = int(np.ceil(nchan / max(nblk, 1.0)))
-        key = [(c // size, int(chan_map[c])) for c in range(nchan)]
-        ch_cut = [0] + [c for c in range(1, nchan) if key[c] != key[c - 1]] + [nchan]
-        rows = [int(r) for r in ind[r0:r1]]
-        for j in range(len(ch_cut) - 1):
-            rec = [ch_cut[j], ch_cut[j + 1]] + rows
-            sizes.append(len(rec))
-            flat.extend(rec)
-    return sizes, flat
-
-
-def bda_map_mode2(uvw, a0, a1, freqs, dfreq, decorr, cell_rad, facet_npix, chan_map):
-    """BDA.Grid / BDA.Degrid map of the phase-based builder (--Comp-BDAMode=2); the same driver
-    quantities as bda_map_mode1 (dPhi from the decorrelation, lmax the FoV radius)."""
-    decorr_in = 1.0 - decorr
-    dphi = np.sqrt(6. * (1. - decorr_in))
-    lmax = cell_rad * facet_npix * (np.sqrt(2.) / 2.)
-    chan_map = np.asarray(chan_map)
-    per_bl = {}
-    for key, ind in _baseline_rows(np.asarray(a0), np.asarray(a1)).items():
-        per_bl[key] = _blocks_bl_mode2(ind, uvw, dfreq, np.asarray(freqs), dphi, lmax, chan_map)
-    return pack(per_bl)
-
-
 def bda_map_per_visibility(a0, a1, nchan):
     """One block per (row, channel): no averaging."""
     per_bl = {}
@@ -242,3 +107,78 @@ def bda_map_factors(a0, a1, nchan, factor_of_baseline, chan_map=None):
                 flat.extend([c0, c1] + rows)
         per_bl[key] = (sizes, flat)
     return pack(per_bl)
+
+
+def _key_ranks(na):
+    """rank[a0, a1]: position of "a0:a1" among the sorted string keys of every ordered antenna
+    pair (the order collectSmearMapping concatenates the baselines in)."""
+    keys = sorted("%d:%d" % (p, q) for p in range(na) for q in range(na) if p != q)
+    rank = np.zeros((na, na), np.int32)
+    for i, k in enumerate(keys):
+        p, q = (int(x) for x in k.split(":"))
+        rank[p, q] = i
+    return rank
+
+
+def bda_map_device(uvw, a0, a1, freqs, dfreq, decorr, cell_rad, facet_npix, chan_map, mode=1, na=None, stream=None):
+    """The BDA map of bda_map_mode1 (mode 1) or bda_map_mode2 (mode 2) built on the device
+    (ddf_bda_build_*), word for word the same int32 map.  uvw float64 [nrow, 3], a0 / a1 int32
+    [nrow], chan_map int32 [nchan]: torch tensors on the GPU (or numpy arrays, uploaded);
+    freqs / dfreq / decorr / cell_rad / facet_npix as the host builders take them.  Returns an
+    int32 tensor on the GPU."""
+    import ctypes as C
+
+    import torch
+
+    from . import abi
+    from ._lib import check, lib
+    from .device import require_gpu, stream_handle, to_device
+    dev = require_gpu()
+    t_uvw = to_device(np.ascontiguousarray(uvw, np.float64), dev) if not isinstance(uvw, torch.Tensor) else uvw
+    t_a0 = to_device(np.ascontiguousarray(a0, np.int32), dev) if not isinstance(a0, torch.Tensor) else a0
+    t_a1 = to_device(np.ascontiguousarray(a1, np.int32), dev) if not isinstance(a1, torch.Tensor) else a1
+    t_cm = to_device(np.ascontiguousarray(chan_map, np.int32), dev) if not isinstance(chan_map, torch.Tensor) else chan_map
+    for t, dt in ((t_uvw, torch.float64), (t_a0, torch.int32), (t_a1, torch.int32), (t_cm, torch.int32)):
+        if t.dtype != dt or not t.is_contiguous() or t.device.type != "cuda":
+            raise TypeError("uvw float64, a0 / a1 / chan_map int32, contiguous on the GPU")
+    nrow = int(t_uvw.shape[0])
+    freqs = np.asarray(freqs, np.float64)
+    if na is None:
+        na = int(max(int(t_a0.max()), int(t_a1.max()))) + 1 if nrow else 1
+    decorr_in = 1.0 - decorr
+    dphi = np.sqrt(6. * (1. - decorr_in))
+    nu0 = np.max(freqs)
+    s = abi.ddf_bda_spec()
+    s.mode, s.nchan, s.na = int(mode), int(freqs.size), int(na)
+    s.dfreq = float(dfreq)
+    if mode == 1:
+        fov_deg = cell_rad * facet_npix * (np.sqrt(2.) / 2.) * 180. / np.pi
+        lrad = fov_deg * np.pi / 180
+        s.duv_cut = float(C_BDA * dphi / (np.pi * lrad * nu0))
+        s.k_dnu = float((C_BDA / np.pi) * dphi)
+        s.l = float(lrad)
+    elif mode == 2:
+        lmax = cell_rad * facet_npix * (np.sqrt(2.) / 2.)
+        s.lmax = float(lmax)
+        s.nmax = float(abs(np.sqrt(1.0 - lmax ** 2) - 1.0))
+        s.k_phase = float(2.0 * np.pi * nu0 / C_BDA)
+        s.k_span = float((C_BDA / (2.0 * np.pi)) * dphi)
+        s.dphi = float(dphi)
+        s.per_row = 0 if (C_BDA * dphi / (2.0 * np.pi * nu0)) else 1
+    else:
+        raise ValueError("BDA mode must be 1 or 2")
+    rank = _key_ranks(int(na))
+    st = C.c_void_p(stream_handle(stream))
+    L = lib()
+    b = C.c_void_p()
+    nw = C.c_int64()
+    check(L.ddf_bda_build_create(C.byref(s), C.c_void_p(t_uvw.data_ptr()), C.c_void_p(t_a0.data_ptr()),
+                                 C.c_void_p(t_a1.data_ptr()), nrow, C.c_void_p(t_cm.data_ptr()),
+                                 rank.ctypes.data_as(C.c_void_p), C.byref(b), C.byref(nw), st))
+    try:
+        out = torch.empty(int(nw.value), dtype=torch.int32, device=dev)
+        check(L.ddf_bda_build_write(b, C.c_void_p(out.data_ptr()), out.numel(), st))
+        torch.cuda.current_stream(dev).synchronize() if stream is None else stream.synchronize()
+    finally:
+        L.ddf_bda_build_destroy(b)
+    return out

@@ -92,9 +92,11 @@ def cfg1(noise=0.01, log=print):
 
 
 def cfg2(nfx=9, image_npix=4096, ntime=600, nchan=64, nant=64, sort_by_baseline=True,
-         with_jones=True, with_vis=True, device="cpu", log=print, time_frac=1.0):
+         with_jones=True, with_vis=True, device="cpu", log=print, time_frac=1.0, bda_builder=None):
     """The 81-facet MeerKAT-like workload.  `time_frac` < 1 keeps the first fraction of the
-    timesteps (for bounded CPU samples); everything else is unchanged."""
+    timesteps (for bounded CPU samples); everything else is unchanged.  `bda_builder(uvw, a0, a1,
+    freqs, dfreq, decorr, cell, facet_npix, chan_map)` returns the int32 BDA map (default: the
+    mode-1 map built on the GPU, bda.bda_map_device)."""
     t0 = time.time()
     freqs = np.linspace(0.9e9, 1.67e9, nchan)
     nt = max(1, int(round(ntime * time_frac)))
@@ -125,7 +127,10 @@ def cfg2(nfx=9, image_npix=4096, ntime=600, nchan=64, nant=64, sort_by_baseline=
         facets.append(f)
     log("cfg2: CF tables in %.1fs" % (time.time() - t0))
     chan_map = np.zeros(nchan, np.int32)
-    bmap = bdamod.bda_map_mode1(obs.uvw, obs.a0, obs.a1, freqs, obs.dfreq, 0.02, cell, nopt, chan_map)
+    if bda_builder is None:
+        bmap = bdamod.bda_map_device(obs.uvw, obs.a0, obs.a1, freqs, obs.dfreq, 0.02, cell, nopt, chan_map).cpu().numpy()
+    else:
+        bmap = bda_builder(obs.uvw, obs.a0, obs.a1, freqs, obs.dfreq, 0.02, cell, nopt, chan_map)
     log("cfg2: BDA map (%d blocks, %d words) in %.1fs" % (bdamod.nblocks_of(bmap), bmap.size, time.time() - t0))
     jones = None
     if with_jones:

@@ -58,10 +58,10 @@ __global__ __launch_bounds__(256) void roll2(const float2* src, float2* dst, int
   }
 }
 
+// A cached rocFFT plan is read-only once made, so calls on several streams or host threads may
+// share it; each call brings its own execution info, work buffer and staging buffer.
 struct FFTPlan {
   rocfft_plan plan = nullptr;
-  rocfft_execution_info info = nullptr;
-  void* work = nullptr;
   size_t work_size = 0;
 };
 
@@ -89,14 +89,9 @@ int get_plan(int n, int64_t nplanes, int inverse, FFTPlan** out) {
                          inverse ? rocfft_transform_type_complex_inverse : rocfft_transform_type_complex_forward,
                          rocfft_precision_single, 2, lengths, size_t(nplanes), nullptr) != rocfft_status_success)
     return ifail(DDF_EHIP, "rocfft_plan_create failed");
-  if (rocfft_plan_get_work_buffer_size(P.plan, &P.work_size) != rocfft_status_success)
+  if (rocfft_plan_get_work_buffer_size(P.plan, &P.work_size) != rocfft_status_success) {
+    (void)rocfft_plan_destroy(P.plan);
     return ifail(DDF_EHIP, "rocfft work size query failed");
-  if (rocfft_execution_info_create(&P.info) != rocfft_status_success)
-    return ifail(DDF_EHIP, "rocfft_execution_info_create failed");
-  if (P.work_size) {
-    HIP_OK(hipMalloc(&P.work, P.work_size));
-    if (rocfft_execution_info_set_work_buffer(P.info, P.work, P.work_size) != rocfft_status_success)
-      return ifail(DDF_EHIP, "rocfft work buffer failed");
   }
   auto res = g_fft_plans.emplace(key, P);
   *out = &res.first->second;
@@ -321,6 +316,19 @@ struct TmpFree {
 
 }  // namespace
 
+// Per-call resources of ddf_fft2c, released on every return path (stream-ordered frees).
+struct FFTCall {
+  hipStream_t st;
+  void* tmp = nullptr;
+  void* work = nullptr;
+  rocfft_execution_info info = nullptr;
+  ~FFTCall() {
+    if (info) (void)rocfft_execution_info_destroy(info);
+    if (work) (void)hipFreeAsync(work, st);
+    if (tmp) (void)hipFreeAsync(tmp, st);
+  }
+};
+
 extern "C" int ddf_fft2c(float* planes, int32_t n, int64_t nplanes, int32_t inverse, double scale, void* stream) {
   if (!planes || n <= 0 || nplanes <= 0) return ifail(DDF_EINVAL, "bad FFT arguments");
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -328,22 +336,28 @@ extern "C" int ddf_fft2c(float* planes, int32_t n, int64_t nplanes, int32_t inve
   int rc = get_plan(n, nplanes, inverse, &P);
   if (rc) return rc;
   const int64_t total = int64_t(n) * n * nplanes;
-  void* tmp = nullptr;
-  HIP_OK(hipMallocAsync(&tmp, size_t(total) * sizeof(float2), st));
+  FFTCall call{st};
+  HIP_OK(hipMallocAsync(&call.tmp, size_t(total) * sizeof(float2), st));
+  if (rocfft_execution_info_create(&call.info) != rocfft_status_success)
+    return ifail(DDF_EHIP, "rocfft_execution_info_create failed");
+  if (P->work_size) {
+    HIP_OK(hipMallocAsync(&call.work, P->work_size, st));
+    if (rocfft_execution_info_set_work_buffer(call.info, call.work, P->work_size) != rocfft_status_success)
+      return ifail(DDF_EHIP, "rocfft work buffer failed");
+  }
+  if (rocfft_execution_info_set_stream(call.info, st) != rocfft_status_success) return ifail(DDF_EHIP, "rocfft stream");
   float2* a = reinterpret_cast<float2*>(planes);
-  float2* t = static_cast<float2*>(tmp);
+  float2* t = static_cast<float2*>(call.tmp);
   // ifftshift: t[j] = a[(j + n/2) % n]
   hipLaunchKernelGGL(roll2, dim3(blocks_for(total)), dim3(256), 0, st, a, t, n, nplanes, n / 2, 1.f);
   HIP_OK(hipGetLastError());
-  if (rocfft_execution_info_set_stream(P->info, st) != rocfft_status_success) return ifail(DDF_EHIP, "rocfft stream");
-  void* bufs[1] = {tmp};
-  if (rocfft_execute(P->plan, bufs, nullptr, P->info) != rocfft_status_success)
+  void* bufs[1] = {call.tmp};
+  if (rocfft_execute(P->plan, bufs, nullptr, call.info) != rocfft_status_success)
     return ifail(DDF_EHIP, "rocfft_execute failed");
   // fftshift: a[k] = t[(k + (n+1)/2) % n], with the normalisation
   double norm = inverse ? scale : scale / (double(n) * double(n));
   hipLaunchKernelGGL(roll2, dim3(blocks_for(total)), dim3(256), 0, st, t, a, n, nplanes, (n + 1) / 2, float(norm));
   HIP_OK(hipGetLastError());
-  HIP_OK(hipFreeAsync(tmp, st));
   return DDF_OK;
 }
 

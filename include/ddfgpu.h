@@ -279,6 +279,32 @@ int ddf_plan_timing(ddf_plan* plan, int32_t enable);
 int ddf_plan_pass_ms(ddf_plan* plan, float* ms, int32_t n);
 int ddf_plan_destroy(ddf_plan* plan);
 
+/* ---- BDA block maps built on the device ------------------------------------------------------
+ * Replaces ClassSmearMapping.computeSmearMappingInBackground + collectSmearMapping
+ * (DDFacet/Data/ClassSmearMapping.py:64-135): one job per baseline running
+ * GiveBlocksRowsListBL_old (mode 1, :496-625, the default BDAMode) or the phase-based builder
+ * (mode 2, :348-491, as restated in ddfacet_amd/bda.py), collected in lexicographic order of the
+ * "a0:a1" keys into the int32 word format [NTot_lo, NTot_hi, sizes..., (chStart, chEnd, rows...)...].
+ * The scalars are the builder's, formed on the host in its operation order:
+ *   mode 1: duv_cut = C dPhi / (pi l nu0), k_dnu = (C / pi) dPhi, l, dfreq (C = 3e8);
+ *   mode 2: k_phase = 2 pi nu0 / C, lmax, nmax = |sqrt(1 - lmax^2) - 1|, k_span = (C / (2 pi)) dPhi,
+ *           dphi, dfreq, per_row = (C dPhi / (2 pi nu0) == 0).
+ * uvw f64 [nrow][3], a0 / a1 int32 [nrow], chan_map int32 [nchan]: device pointers; rank: host
+ * int32 [na][na], the position of "a0:a1" among the sorted keys of every ordered pair a0 != a1.
+ * create sorts the rows by baseline and counts the map (it synchronises `stream` to return
+ * nwords); write fills a device buffer of at least nwords int32 words.                          */
+typedef struct ddf_bda_spec {
+  int32_t mode, nchan, na, per_row;
+  double duv_cut, k_dnu, l, dfreq;
+  double k_phase, lmax, nmax, k_span, dphi;
+} ddf_bda_spec;
+typedef struct ddf_bda_builder ddf_bda_builder;
+int ddf_bda_build_create(const ddf_bda_spec* spec, const double* uvw, const int32_t* a0, const int32_t* a1,
+                         int64_t nrow, const int32_t* chan_map, const int32_t* rank, ddf_bda_builder** builder,
+                         int64_t* nwords, void* stream);
+int ddf_bda_build_write(ddf_bda_builder* builder, int32_t* out, int64_t n, void* stream);
+int ddf_bda_build_destroy(ddf_bda_builder* builder);
+
 /* ---- on-device CF tables ----------------------------------------------------------------------
  * Replaces the w-plane loop of ModCF.ClassWTermModified (DDFacet/Imager/ModCF.py:444-597) for
  * many facets at once: per facet f and plane i of support S_i the S x S screen

@@ -18,6 +18,7 @@ from the builder's rules by hand:
 import numpy as np
 
 from ddfacet_amd import bda
+from oracle import bda as obda
 
 C = 3e8  # the builder's speed of light (ClassSmearMapping.py:508)
 
@@ -58,7 +59,7 @@ def _mode1_one_baseline(nrows, chan_map=None, R=1000.0, dtheta=0.01, nch=2.5, cu
     lrad = 0.01
     dphi = duv_cut * np.pi * lrad * nu0 / C
     chan_map = np.zeros(8, np.int32) if chan_map is None else np.asarray(chan_map, np.int32)
-    sizes, flat = bda._blocks_bl_old(np.arange(nrows), uvw, dfreq, freqs, dphi, lrad, chan_map)
+    sizes, flat = obda._blocks_bl_old(np.arange(nrows), uvw, dfreq, freqs, dphi, lrad, chan_map)
     return bda.pack({"0:1": (sizes, flat)})
 
 
@@ -115,7 +116,7 @@ def test_mode1_driver_decorr_and_fov_and_baseline_order():
     uvw = np.concatenate([u1, u2, u3])
     a0 = np.concatenate([a01, a02, a03])
     a1 = np.concatenate([a11, a12, a13])
-    m = bda.bda_map_mode1(uvw, a0, a1, freqs, dfreq, decorr, cell, npix, np.zeros(8, np.int32))
+    m = obda.bda_map_mode1(uvw, a0, a1, freqs, dfreq, decorr, cell, npix, np.zeros(8, np.int32))
     got = _blocks(m)
     # "10:11" (rows 7-10) first: blocks {7,8,9}, {10}; then "2:10" (rows 0-6) as in the 7-row case
     want = []
@@ -134,7 +135,7 @@ def test_mode1_row_step_is_the_distance_to_the_next_row():
     nu0, dfreq, lrad = 1e9, 1e7, 0.01
     duv_cut = 15.0
     dphi = duv_cut * np.pi * lrad * nu0 / C
-    sizes, flat = bda._blocks_bl_old(np.arange(6), uvw, dfreq, _freqs(nu0, dfreq), dphi, lrad, np.zeros(8, np.int32))
+    sizes, flat = obda._blocks_bl_old(np.arange(6), uvw, dfreq, _freqs(nu0, dfreq), dphi, lrad, np.zeros(8, np.int32))
     m = bda.pack({"0:1": (sizes, flat)})
     # nch = 15 * 1e9 / (1e6 * 1e7) < 1 -> single channels
     want = [(c, c + 1, [0, 1, 2]) for c in range(8)] + [(c, c + 1, [3, 4, 5]) for c in range(8)]
@@ -153,7 +154,7 @@ def _mode2(uvw, dphi, lmax, nchan=8, dfreq=None, chan_map=None):
     dfreq = NU1 / 100.0 if dfreq is None else dfreq
     chan_map = np.zeros(nchan, np.int32) if chan_map is None else np.asarray(chan_map, np.int32)
     ind = np.arange(uvw.shape[0])
-    sizes, flat = bda._blocks_bl_mode2(ind, uvw, dfreq, freqs, dphi, lmax, chan_map)
+    sizes, flat = obda._blocks_bl_mode2(ind, uvw, dfreq, freqs, dphi, lmax, chan_map)
     return _blocks(bda.pack({"0:1": (sizes, flat)}))
 
 
@@ -217,7 +218,7 @@ def test_mode2_driver_decorr_fov_and_baseline_order():
     a0 = np.array([2] * 7 + [10] + [3, 3], np.int32)
     a1 = np.array([10] * 7 + [11] + [3, 3], np.int32)
     freqs = np.linspace(0.5 * NU1, NU1, 8)
-    m = bda.bda_map_mode2(uvw, a0, a1, freqs, NU1 / 100.0, decorr, cell, npix, np.zeros(8, np.int32))
+    m = obda.bda_map_mode2(uvw, a0, a1, freqs, NU1 / 100.0, decorr, cell, npix, np.zeros(8, np.int32))
     got = _blocks(m)
     # "10:11" (row 7, span 2.6) first, then "2:10" (rows 0-6) as in the 7-row case
     want = [(c, c + 2, [7]) for c in (0, 2, 4, 6)]

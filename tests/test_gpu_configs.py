@@ -43,11 +43,12 @@ def test_plain_single_facet_workload_against_reference():
     from oracle import imaging as ref
     print("plain single-facet workload", flush=True)
     P = configs.cfg1(log=lambda *a: None)
-    model = model_grid(P, 7) * np.float32(1e-3)
-
+    model = model_grid(P, 7)
+    # predictions are compared from zero visibilities (degridding subtracts the model), so that
+    # the FP32 rounding of the observed visibilities does not enter the comparison
     g_hip, s_hip = P.new_grid(), np.zeros((P.nband, P.npol))
     shim.pyGridderWPol32(*P.grid_args(0, g_hip, s_hip))
-    v_hip = P.vis.copy()
+    v_hip = np.zeros_like(P.vis)
     shim.pyDeGridderWPol(*P.degrid_args(0, model, v_hip))
 
     fs = DeviceFacetSet(P)
@@ -55,7 +56,7 @@ def test_plain_single_facet_workload_against_reference():
     fs.plan_batched()
     fs.zero_grids()
     fs.grid_batched()
-    vis_b = fs.t["vis"].clone()
+    vis_b = torch.zeros_like(fs.t["vis"])
     fs.degrid_batched(vis=vis_b)
     torch.cuda.synchronize()
     g_bat, s_bat = fs.grids[0].cpu().numpy(), fs.sumwt[0].cpu().numpy()
@@ -65,11 +66,11 @@ def test_plain_single_facet_workload_against_reference():
     for label, grid_fn, degrid_fn in _grid_oracles():
         g_ref, s_ref = P.new_grid(), np.zeros((P.nband, P.npol))
         grid_fn(*P.grid_args(0, g_ref, s_ref))
-        v_ref = P.vis.copy()
+        v_ref = np.zeros_like(P.vis)
         degrid_fn(*P.degrid_args(0, model, v_ref))
         for path, g, s, v in (("plain", g_hip, s_hip, v_hip), ("batched", g_bat, s_bat, v_bat)):
             eg = rel_l2(g, g_ref)
-            ev = rel_l2(P.vis - v, P.vis - v_ref)
+            ev = rel_l2(v, v_ref)
             print("%s vs %s: grid rel L2 %.3e, degridded vis rel L2 %.3e" % (path, label, eg, ev), flush=True)
             assert eg <= 1e-5 and ev <= 1e-5, (path, label, eg, ev)
             assert np.array_equal(g != 0, g_ref != 0), (path, label)

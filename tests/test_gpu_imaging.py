@@ -24,7 +24,7 @@ def test_fft2c(n, inverse):
     assert rel_l2(t.cpu().numpy(), want) <= 1e-5
 
 
-def _setup(seed, m=151, nfx=3, n=75, nband=2, npol=2, jones=True):
+def _setup(seed, m=153, nfx=3, n=75, nband=2, npol=2, jones=True):
     from ddfacet_amd.imaging import FacetGeometry
     from ddfacet_amd.wterm import SpheMachine
     geom = FacetGeometry(m, nfx, m // nfx, n, 1e-5, mixing_width=4.0)

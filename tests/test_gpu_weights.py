@@ -66,3 +66,36 @@ def test_weights_index_is_bit_exact():
     torch.cuda.synchronize()
     want = ow.uv_to_index(uvw, w, freqs, cell, npix, npixx, 4, xymax, chan_band=cb)
     assert np.array_equal(idx.cpu().numpy(), want)
+
+
+def test_scatter_add_long_runs_of_one_cell():
+    """Rows in baseline-time order put runs of consecutive rows in one weight-grid cell: here
+    every uvw row is repeated 8-100 times (with sub-cell jitter), so the run-summed scatter-add
+    sees runs of 3 to 100 lanes, runs cut by zero weights, runs ending at lane 63 and runs
+    crossing the 64-row tile edge.  Grid and final weights against the oracle."""
+    from ddfacet_amd import weights as dw
+    rng = np.random.default_rng(31)
+    base = rng.uniform(-2000, 2000, (70, 3))
+    reps = rng.integers(8, 101, base.shape[0])
+    uvw = np.repeat(base, reps, axis=0) + rng.normal(0, 1e-3, (int(reps.sum()), 3))
+    nrow, nchan = uvw.shape[0], 8
+    freqs = np.linspace(0.9e9, 1.67e9, nchan)
+    w = rng.uniform(0.5, 1.5, (nrow, nchan)).astype(np.float32)
+    w[rng.random((nrow, nchan)) < 0.15] = 0
+    uvmax = float(np.abs(uvw[:, :2]).max() * freqs.max() / C0)
+    cell = 97.0
+    xymax, npixx, npix = dw.grid_geometry(uvmax, cell)
+    idx = ow.uv_to_index(uvw, np.ones_like(w), freqs, cell, npix, npixx, 1, xymax)
+    same = idx[1:] == idx[:-1]                        # [row, chan]: row r + 1 continues row r's cell
+    runs3 = sum(int(np.sum(np.convolve(same[:, c].astype(int), [1, 1], "valid") == 2)) for c in range(nchan))
+    cross = int(np.sum(same[63::64]))                 # row 64 k + 63 -> 64 (k + 1) in one cell
+    assert runs3 > 1000 and cross > 10, (runs3, cross)
+    for weighting in ("uniform", "briggs"):
+        got, grid = dw.imaging_weights(uvw, w, freqs, cell, uvmax, weighting=weighting, robust=0.5, return_grid=True)
+        torch.cuda.synchronize()
+        want, wgrid = ow.imaging_weights(uvw, w, freqs, cell, xymax, npixx, npix, weighting=weighting, robust=0.5)
+        np.testing.assert_allclose(grid.cpu().numpy(), wgrid, rtol=1e-12, atol=0)
+        got = got.cpu().numpy()
+        nz = w != 0
+        np.testing.assert_allclose(got[nz], want[nz], rtol=2.4e-7, atol=0)
+        np.testing.assert_array_equal(got[~nz], want[~nz])
