@@ -172,6 +172,13 @@ class ddf_cf_spec(C.Structure):
 PRECISION = {"mixed": 0, "fp64": 1, "fp32": 2}
 
 
+class ddf_flag_select(C.Structure):
+    _fields_ = [("na", C.c_int32), ("use_uv_range", C.c_int32), ("use_time_range", C.c_int32), ("pad", C.c_int32),
+                ("uv_min_m2", C.c_double), ("uv_max_m2", C.c_double), ("time_min", C.c_double),
+                ("time_max", C.c_double), ("time_offset", C.c_double), ("ant_threshold", C.c_double),
+                ("named_antennas", C.POINTER(C.c_uint8))]
+
+
 class ddf_bda_spec(C.Structure):
     _fields_ = [("mode", C.c_int32), ("nchan", C.c_int32), ("na", C.c_int32), ("per_row", C.c_int32),
                 ("duv_cut", C.c_double), ("k_dnu", C.c_double), ("l", C.c_double), ("dfreq", C.c_double),
@@ -196,6 +203,10 @@ EXTRA_SIGNATURES = {
                              C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int64), C.c_void_p],
     "ddf_bda_build_write": [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p],
     "ddf_bda_build_destroy": [C.c_void_p],
+    "ddf_sort_baseline_time": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p],
+    "ddf_gather_rows": [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p],
+    "ddf_flag_selection": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
+                           C.POINTER(ddf_flag_select), C.c_void_p, C.c_void_p, C.c_void_p],
     "ddf_cf_build": [C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p],
     "ddf_plan_timing": [C.c_void_p, C.c_int32],
     "ddf_plan_pass_ms": [C.c_void_p, C.POINTER(C.c_float), C.c_int32],

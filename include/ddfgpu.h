@@ -279,6 +279,36 @@ int ddf_plan_timing(ddf_plan* plan, int32_t enable);
 int ddf_plan_pass_ms(ddf_plan* plan, float* ms, int32_t n);
 int ddf_plan_destroy(ddf_plan* plan);
 
+/* ---- chunk staging --------------------------------------------------------------------------
+ * The baseline-time sort of ClassMS.ReadData (DDFacet/Data/ClassMS.py:575-586):
+ * index[k] = the row of rank k in sorted(zip(A0, A1, TIME, range(nrow))) (antennas < 65536).
+ * a0 / a1 int32, times f64, index int32: device arrays of nrow entries.                         */
+int ddf_sort_baseline_time(const int32_t* a0, const int32_t* a1, const double* times, int64_t nrow, int32_t* index,
+                           void* stream);
+/* dst[k] = src[index[k]] for rows of row_bytes bytes (applies the sort index to uvw, data, flags,
+ * weights, ... as ReadData does with sort_index); src and dst must not overlap.                  */
+int ddf_gather_rows(const void* src, void* dst, int64_t nrow, int64_t row_bytes, const int32_t* index, void* stream);
+/* The selections of ClassMS.UpdateFlags (ClassMS.py:1169-1233) that follow ddf_prepare_flags, on
+ * flags uint8 [nrow][nchan][ncorr] equalised over the correlations, in place:
+ *   per antenna the fraction of flagged (row, channel) cells of its rows (counted before the cuts
+ *   below, as the reference counts flags1) -> ant_frac f64 [na] (device);
+ *   rows with u^2 + v^2 outside [uv_min_m2, uv_max_m2] (use_uv_range; the reference's
+ *   UVRangeKm d -> d^2 1e6), rows with times - time_offset outside [time_min, time_max]
+ *   (use_time_range; TimeRange on unix times, time_offset = MJD-seconds to unix seconds) flagged;
+ *   every row of an antenna with a fraction above ant_threshold (0.9 in the reference) or named
+ *   in named_antennas (host uint8 [na]: DistMaxToCore, FlagAnts; may be NULL) flagged;
+ *   ant_flagged uint8 [na] (device) receives the antennas flagged.                               */
+typedef struct ddf_flag_select {
+  int32_t na, use_uv_range, use_time_range, pad;
+  double uv_min_m2, uv_max_m2;
+  double time_min, time_max, time_offset;
+  double ant_threshold;
+  const uint8_t* named_antennas;
+} ddf_flag_select;
+int ddf_flag_selection(uint8_t* flags, const double* uvw, const double* times, const int32_t* a0, const int32_t* a1,
+                       int64_t nrow, int32_t nchan, int32_t ncorr, const ddf_flag_select* sel, double* ant_frac,
+                       uint8_t* ant_flagged, void* stream);
+
 /* ---- BDA block maps built on the device ------------------------------------------------------
  * Replaces ClassSmearMapping.computeSmearMappingInBackground + collectSmearMapping
  * (DDFacet/Data/ClassSmearMapping.py:64-135): one job per baseline running

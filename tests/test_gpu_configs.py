@@ -87,15 +87,21 @@ def test_plain_single_facet_workload_against_reference():
     sphe = np.float32(P.facets[0].sphe).astype(np.float64)
     d = ref.grid_to_im(g_hip, int(P.facets[0].winfos[3]))[0, 0].real.astype(np.float64) / sphe / s_hip[0, 0]
     img = d[::-1, :].T
+    # the sources in the inner half of the image (at their nearest pixels) and 64 random pixels there
+    sky = P.extra["sky"]
+    Xs = np.rint(sky.l / cell + P.npix // 2).astype(int)
+    Ys = np.rint(sky.m / cell + P.npix // 2).astype(int)
+    inner = (np.abs(Xs - P.npix // 2) < P.npix // 4) & (np.abs(Ys - P.npix // 2) < P.npix // 4)
     rng = np.random.default_rng(0)
-    X, Y = rng.integers(128, 384, 64), rng.integers(128, 384, 64)
+    X = np.concatenate([Xs[inner], rng.integers(128, 384, 64)])
+    Y = np.concatenate([Ys[inner], rng.integers(128, 384, 64)])
     l, m = dft.image_pixel_lm(X, Y, P.npix, cell)
     w = np.where(kept, P.weights, 0).astype(np.float32)
     want = dft.dirty(P.uvw, P.freqs, 0.5 * (P.vis[:, :, 0] + P.vis[:, :, 3]).astype(np.complex128), w,
                      P.flags[:, :, 0], l, m)
     err = np.abs(img[X, Y] - want).max() / np.abs(want).max()
-    print("gridded fraction %.3f; dirty image vs FP64 DFT (64 inner pixels): max error / peak %.3e"
-          % (kept.mean(), err), flush=True)
+    print("gridded fraction %.3f; dirty image vs FP64 DFT (%d inner sources + 64 inner pixels): max error / peak "
+          "%.3e" % (kept.mean(), int(inner.sum()), err), flush=True)
     assert err < 0.1
 
 

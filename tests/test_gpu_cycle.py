@@ -157,7 +157,9 @@ def test_autograd_gradient_vs_finite_differences():
             d = (rp.real.double() ** 2 + rp.imag.double() ** 2) - (rm.real.double() ** 2 + rm.imag.double() ** 2)
             fd = float((w[..., None] * d).sum()) / (2 * h)
         errs.append(abs(fd - g[a, b]) / max(abs(fd), 1e-3 * np.abs(g).max()))
-    print("autograd vs central differences: max rel err %.3e" % max(errs))
+    k = int(np.argmax(errs))
+    print("autograd vs central differences: max rel err %.3e (pixel %s, |g| / max|g| %.2e)"
+          % (errs[k], pix[k], abs(g[pix[k]]) / np.abs(g).max()))
     assert max(errs) <= 1e-4
 
 
