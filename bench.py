@@ -537,6 +537,92 @@ def run_bda(a):
     print(json.dumps(out))
 
 
+def run_pipeline(a, nchunks=4):
+    """Major cycles over a stream of chunks whose visibilities arrive from (pinned) host memory:
+    the upload of chunk n + 1 (copy stream) overlapped with the grid + degrid cycle of chunk n
+    (compute stream), as the reference loads the next chunk while the imager works
+    (ClassDeconvMachine.py:1320-1337).  Chunks: 300 timesteps of the 81-facet MeerKAT-like
+    workload each (the metadata, BDA maps and plan of one such chunk, the visibilities of each
+    chunk its own); timed serial (upload, then cycle) and pipelined."""
+    import torch
+    from ddfacet_amd import configs
+    from ddfacet_amd.chunk import ChunkPipeline
+    from ddfacet_amd.facets import DeviceFacetSet
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    P = configs.cfg2(with_vis=False, log=log, time_frac=0.5)
+    R, C = P.uvw.shape[0], P.freqs.shape[0]
+    P.vis = torch.zeros((R, C, 4), dtype=torch.complex64, device=dev)
+    fs = DeviceFacetSet(P, device=dev)
+    gen = torch.Generator(device=dev).manual_seed(9)
+    fs.models.copy_(torch.randn(fs.models.shape, generator=gen, device=dev, dtype=torch.complex64) * 1e-3)
+    fs.plan_batched()
+    rng = np.random.default_rng(10)
+    host = []
+    for k in range(nchunks):
+        v = torch.empty((R, C, 4), dtype=torch.complex64).pin_memory()
+        v.view(torch.float32).copy_(torch.from_numpy(rng.standard_normal((R, C, 8), dtype=np.float32)))
+        host.append({"vis": v})
+    nbytes = host[0]["vis"].numel() * 8
+    compute = torch.cuda.current_stream()
+
+    def cycle(vis):
+        fs.zero_grids()
+        fs.grid_batched(vis=vis, stream=compute)
+        fs.degrid_batched(vis=vis, stream=compute)
+
+    def serial():
+        for k in range(nchunks):
+            fs.t["vis"].copy_(host[k]["vis"], non_blocking=True)
+            cycle(fs.t["vis"])
+
+    pipe = ChunkPipeline(dev)
+
+    def pipelined():
+        pipe.submit(0, host[0])
+        for k in range(nchunks):
+            if k + 1 < nchunks:
+                pipe.submit(k + 1, host[k + 1])
+            d = pipe.wait(k, compute)
+            cycle(d["vis"])
+            pipe.release(k, compute)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / a.steps
+
+    # the parts alone
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        fs.t["vis"].copy_(host[0]["vis"], non_blocking=True)
+    torch.cuda.synchronize()
+    ms_copy = (time.perf_counter() - t0) * 1e3 / a.steps
+    ms_cycle = timed(lambda: cycle(fs.t["vis"]))
+    ms_serial = timed(serial)
+    ms_pipe = timed(pipelined)
+    nvis = R * C * nchunks
+    out = {
+        "metric": "vis/sec grid+degrid over a stream of chunks uploaded from host memory, 81 facets",
+        "value": nvis / (ms_pipe * 1e-3), "unit": "vis/s", "n_gpus": 1, "steps": a.steps, "warmup": 1,
+        "ms_per_step": ms_pipe, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 (fp32 grids, fp64 phase base)", "data": "synthetic",
+        "config": {"workload": "%d chunks of 300 x 8 s of the 81-facet MeerKAT-like workload (64 antennas, 64 ch, "
+                               "81 facets), visibilities in pinned host memory" % nchunks,
+                   "rows_per_chunk": R, "channels": C, "chunks": nchunks, "upload_bytes_per_chunk": nbytes},
+        "upload_ms_per_chunk": ms_copy, "upload_GBps": nbytes / (ms_copy * 1e-3) / 1e9,
+        "cycle_ms_per_chunk": ms_cycle, "serial_ms_per_step": ms_serial, "pipelined_ms_per_step": ms_pipe,
+        "hidden_fraction_of_uploads": (ms_serial - ms_pipe) / (nchunks * ms_copy),
+    }
+    fs.destroy_plan()
+    print(json.dumps(out))
+
+
 def run_autograd(a, gd_steps=20):
     """chi^2 gradient descent through the differentiable degridder (forward: model cube ->
     visibilities, backward: the hand-written adjoint), 25 facets x 4 bands."""
@@ -724,7 +810,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--facets", type=int, default=0, help="use only the first N facets (debug)")
     ap.add_argument("--workload", default="meerkat81",
-                    choices=["meerkat81", "single512", "uneven121", "autograd25", "weights", "flags", "bda"],
+                    choices=["meerkat81", "single512", "uneven121", "autograd25", "weights", "flags", "bda",
+                             "pipeline"],
                     help="meerkat81: the 81-facet MeerKAT-like cycle (the headline line); single512: the "
                          "plain single-facet workload, CPU and GPU plain / batched; uneven121: "
                          "121 facets x 4 bands with baseline-dependent averaging, plain per-facet "
@@ -732,7 +819,8 @@ def main():
                          "autograd25: 20 chi^2 gradient-descent steps through the differentiable "
                          "degridder on 25 facets x 4 bands; weights: Briggs imaging weights of the "
                          "81-facet workload's chunk; flags: flag preparation of that chunk; bda: the "
-                         "chunk's BDA block map built on the device")
+                         "chunk's BDA block map built on the device; pipeline: chunks uploaded from host memory "
+                         "on a copy stream while the previous chunk's cycle runs")
     ap.add_argument("--tile", type=int, default=0, help="uv tile edge of the batched plan (0: the plan default)")
     ap.add_argument("--blocks-per-item", type=int, default=0,
                     help="BDA blocks per pass-A work item of the batched plan (0: the plan default)")
@@ -776,6 +864,8 @@ def main():
         return run_flags(a)
     if a.workload == "bda":
         return run_bda(a)
+    if a.workload == "pipeline":
+        return run_pipeline(a)
     from ddfacet_amd import configs
     t0 = time.time()
     # geometry, maps, CFs and Jones; the CPU baseline runs before the GPU is touched, so with it

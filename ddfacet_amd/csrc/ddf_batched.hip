@@ -270,6 +270,26 @@ __global__ void block_vis_counts(const VisP v, int32_t* cnt) {
   }
 }
 
+// Per-block descriptor (VisP::bdesc) for the per-block passes: record offset, row count, channel
+// range and, when the block's rows are consecutive rows of the chunk (a baseline's time block in
+// a baseline-time sorted chunk), the first row, so that a pass addresses the visibilities without
+// reading the map's row list.
+__global__ void plan_block_desc(const VisP v, uint4* desc) {
+  for (int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; b < v.nblocks;
+       b += int64_t(gridDim.x) * blockDim.x) {
+    const BlockRec br = read_block(v, b);
+    const int64_t s = br.rows - gptr(v.bda) - 2;
+    bool consec = br.nr > 0;
+    const int64_t r0 = br.nr > 0 ? int64_t(br.rows[0]) : 0;
+    for (int r = 0; r < br.nr && consec; ++r) {
+      const int64_t row = br.rows[r];
+      consec = row == r0 + r && row >= 0 && row < v.nrow;
+    }
+    desc[b] = make_uint4(uint32_t(s), uint32_t(br.nr) | (consec ? 0x80000000u : 0u),
+                         uint32_t(br.ch0) | (uint32_t(br.ch1) << 16), uint32_t(consec ? r0 : 0));
+  }
+}
+
 __global__ void reset_counter(uint32_t* c, int n) {
   if (threadIdx.x < n) c[threadIdx.x] = 0;
 }
@@ -424,8 +444,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
       const uint64_t rec = place[b * p.nf + fc];
       const bool doit = f < p.nf && place_valid(rec);
       const FacetDev fd = gload(p.facets + fc);
-      BlockRec br = read_block(v, b);
-      const int64_t mid = br.rows[br.nr / 2];
+      const BlockDescRec bdr = read_block_desc(v, b);
+      const BlockRec& br = bdr.br;
+      const int64_t mid = br.nr > 0 ? bdr.row(br.nr / 2) : -1;
       const bool mid_ok = mid >= 0 && mid < v.nrow;
       JonesP jp = p.jp;
       jp.kdir = fd.kms_dir;
@@ -448,7 +469,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
         double su = 0, sv = 0, sw2 = 0;
         int cnt = 0;
         for (int r = 0; r < br.nr; ++r) {
-          const int64_t row = br.rows[r];
+          const int64_t row = bdr.row(r);
           if (row < 0 || row >= v.nrow) continue;
           const DDF_GLOBAL double* q = gptr(v.uvw) + 3 * row;
           for (int c = br.ch0; c < br.ch1; ++c) {
@@ -525,7 +546,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
           const int g = i0 + i;
           int rr, cc;
           wo.at(g, rr, cc);
-          const int64_t srow = br.rows[rr];
+          const int64_t srow = bdr.row(rr);
           float* e = stg + i * kStride;
           if (srow < 0 || srow >= v.nrow) {
             e[kF] = __int_as_float(1);
@@ -557,7 +578,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
             if (g > 0) {
               int pr, pc;
               wo.at(g - 1, pr, pc);
-              const int64_t prow = br.rows[pr];
+              const int64_t prow = bdr.row(pr);
               if (prow >= 0 && prow < v.nrow && !gptr(v.flags)[(size_t(prow) * v.nchan + pc) * v.ncorr]) {
                 int kp[6];
                 key_of(prow, pc, kp);
@@ -1408,6 +1429,14 @@ __global__ __launch_bounds__(256) void degrid_adj_b(BatchParams p, const VisP v,
 
 // ---- host side -----------------------------------------------------------------------------------
 namespace ddf {
+
+int batched_block_desc(const VisP& v, uint4* desc, hipStream_t st) {
+  int64_t nb = (v.nblocks + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(plan_block_desc, dim3(unsigned(nb)), dim3(256), 0, st, v, desc);
+  return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
+}
 
 int batched_place(const BatchParams& p, const VisP& v, bool degrid, uint64_t* place, hipStream_t st) {
   int64_t total = v.nblocks * p.nf;

@@ -23,6 +23,9 @@ struct VisP {
   const uint8_t* sparse;
   int equi;  // channels equidistant (LOptimisation[1])
   double f0, df;  // equidistant channels: first frequency and step (set by the batched plan)
+  // per-block descriptors of the batched plan (null elsewhere): x record offset in bda, y row
+  // count | 1 << 31 when the rows are row0, row0 + 1, ..., z ch0 | ch1 << 16, w row0
+  const uint4* bdesc;
 };
 
 struct BlockRec {
@@ -49,6 +52,27 @@ __device__ __forceinline__ BlockRec read_block(const VisP& v, int64_t b) {
   r.nr = nr;
   r.rows = gptr(v.bda) + s + 2;
   return r;
+}
+
+// Block b from its plan descriptor (one 16-byte load, no dependent loads): the BlockRec of
+// read_block, plus whether its rows are consecutive (row r of the block is then row0 + r).
+struct BlockDescRec {
+  BlockRec br;
+  bool consec;
+  int64_t row0;
+  __device__ __forceinline__ int64_t row(int r) const { return consec ? row0 + r : int64_t(br.rows[r]); }
+};
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ BlockDescRec read_block_desc(const VisP& v, int64_t b) {
+  const u32x4 d = reinterpret_cast<const DDF_GLOBAL u32x4*>(gptr(reinterpret_cast<const uint32_t*>(v.bdesc)))[b];
+  BlockDescRec o;
+  o.br.rows = gptr(v.bda) + int64_t(d.x) + 2;
+  o.br.nr = int(d.y & 0x7FFFFFFFu);
+  o.br.ch0 = int(d.z & 0xFFFFu);
+  o.br.ch1 = int(d.z >> 16);
+  o.consec = (d.y >> 31) != 0;
+  o.row0 = int64_t(d.w);
+  return o;
 }
 
 // Phasor exp(i * freq * angle) at three precisions:
@@ -187,6 +211,7 @@ constexpr int grid_a_stride(int read) { return (grid_a_key_offset(read) + 12 + 3
 
 // host launchers (ddf_batched.hip)
 int batched_place(const BatchParams& p, const VisP& v, bool degrid, uint64_t* place, hipStream_t st);
+int batched_block_desc(const VisP& v, uint4* desc, hipStream_t st);
 int batched_count(const BatchParams& p, int64_t nblocks, const uint64_t* place, uint32_t* counts, hipStream_t st);
 int batched_fill(const BatchParams& p, int64_t nblocks, const uint64_t* place, const int64_t* bin_start,
                  uint32_t* fill, uint32_t* recs, hipStream_t st);

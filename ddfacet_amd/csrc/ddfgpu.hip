@@ -1633,6 +1633,15 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   }
   fill_vis(P->vg, vgrid, sg);
   fill_vis(P->vd, vdegrid, sd);
+  // per-block descriptors of the grid map (grid pass A reads them instead of the row lists)
+  if (vgrid->bda_len >= (int64_t(1) << 32) || vgrid->nchan > 65535)
+    return bail(fail(DDF_EINVAL, "BDA map too long or too many channels for the block descriptors"));
+  {
+    uint4* d = P->alloc<uint4>(size_t(vgrid->nblocks));
+    if (!d) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
+    if (vgrid->nblocks && batched_block_desc(P->vg, d, st)) return bail(fail(DDF_EHIP, "plan_block_desc failed"));
+    P->vg.bdesc = d;
+  }
   P->pg.nblk = vgrid->nblocks;
   P->pd.nblk = vdegrid->nblocks;
   // first frequency and step of equidistant channels (the anchors of ChanPhasor)

@@ -131,7 +131,9 @@ def test_autograd_dot_product_adjoint(nband):
 def test_autograd_gradient_vs_finite_differences():
     """Central differences on 64 pixels of an FP64-precision plan, the loss difference summed per
     visibility in FP64 (chi^2 is quadratic in the image, so any step is free of truncation
-    error)."""
+    error).  Each pixel's error is relative to its derivative or to 1 % of the largest one,
+    whichever is larger: the adjoint accumulates in FP32 grids, ~1e-7 of the largest derivative,
+    which says nothing about a derivative a thousand times smaller."""
     from ddfacet_amd.autograd import DegridOperator, chi2
     P, geom, cyc = _cycle(precision="fp64")
     op = DegridOperator(cyc)
@@ -156,7 +158,7 @@ def test_autograd_gradient_vs_finite_differences():
             rm = op.forward(torch.from_numpy(xm).cuda()) - data
             d = (rp.real.double() ** 2 + rp.imag.double() ** 2) - (rm.real.double() ** 2 + rm.imag.double() ** 2)
             fd = float((w[..., None] * d).sum()) / (2 * h)
-        errs.append(abs(fd - g[a, b]) / max(abs(fd), 1e-3 * np.abs(g).max()))
+        errs.append(abs(fd - g[a, b]) / max(abs(fd), 1e-2 * np.abs(g).max()))
     k = int(np.argmax(errs))
     print("autograd vs central differences: max rel err %.3e (pixel %s, |g| / max|g| %.2e)"
           % (errs[k], pix[k], abs(g[pix[k]]) / np.abs(g).max()))
