@@ -192,6 +192,7 @@ EXTRA_SIGNATURES = {
     "ddf_weights_accumulate": [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p],
     "ddf_weights_briggs": [C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_void_p, C.c_int64, C.c_void_p],
     "ddf_weights_finalize": [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p],
+    "ddf_weights_briggs_noalloc": [C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_void_p],
     "ddf_plan_create": [C.POINTER(ddf_grid_desc), C.POINTER(ddf_vis), C.POINTER(ddf_vis), C.POINTER(ddf_facet),
                         C.POINTER(C.c_void_p), C.c_int32, C.POINTER(ddf_plan_opts), C.POINTER(C.c_void_p),
                         C.c_void_p],

@@ -17,6 +17,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <algorithm>
 #include <cmath>
@@ -786,6 +787,37 @@ namespace ddf {
 void set_last_error(const std::string& msg) { g_err = msg; }
 }  // namespace ddf
 
+// ---- Briggs transform without allocation ------------------------------------------------------
+// The per-workgroup partial sums of the robust transform live in a file-scope device array, one
+// copy per GPU, instead of a caller workspace: for callers that cannot allocate (graph capture,
+// serving loops).  Its address on each device comes from hipGetSymbolAddress, once per device
+// (ddf_init, or the first call on a device); host code uses that address, never the symbol.
+constexpr int kBriggsBlocks = 256;  // workgroups per band of the robust transform
+constexpr int kBriggsMaxBands = 64;
+static __device__ __align__(256) double scratch[kBriggsMaxBands * kBriggsBlocks * 2];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of `scratch` on the current device (resolved once per device).
+int scratch_address(double** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(DDF_EINVAL, "device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(DDF_EHIP, "hipGetSymbolAddress returned a null scratch address");
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return DDF_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int ddf_version(void) { return DDF_ABI_VERSION; }
@@ -795,7 +827,8 @@ const char* ddf_last_error(void) { return g_err.c_str(); }
 int ddf_init(int device) {
   HIP_TRY(hipSetDevice(device));
   HIP_TRY(hipFree(nullptr));
-  return DDF_OK;
+  double* ptr = nullptr;
+  return scratch_address(&ptr);
 }
 
 int ddf_bda_index(const int32_t* bda, int64_t bda_len, int64_t nblocks, int64_t* out, void* stream) {
@@ -985,7 +1018,6 @@ __global__ __launch_bounds__(256) void weights_accumulate_k(double* grid, int64_
 // over a strided share of the band's cells; the second has every workgroup reduce the band's
 // partials in the same order (so each obtains the same avgW = sum g^2 / sum g), then apply
 // g <- 1 + g sSq, sSq = num2 / avgW, to its share, the product rounded before the sum.
-constexpr int kBriggsBlocks = 256;
 
 __global__ __launch_bounds__(256) void weights_briggs_sums_k(const double* grid, int64_t npix, double* part) {
   __shared__ double s2[256], s1[256];
@@ -1111,6 +1143,27 @@ extern "C" int ddf_weights_briggs(double* grid, int nbands, int64_t npix, double
   hipLaunchKernelGGL(weights_briggs_sums_k, grid2, dim3(256), 0, static_cast<hipStream_t>(stream), grid, npix, work);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(weights_briggs_apply_k, grid2, dim3(256), 0, static_cast<hipStream_t>(stream), grid, npix, work,
+                     num2);
+  HIP_TRY(hipGetLastError());
+  return DDF_OK;
+}
+
+extern "C" int ddf_weights_briggs_noalloc(double* grid, int nbands, int64_t npix, double robust, void* stream) {
+  if (nbands < 0 || npix < 0) return fail(DDF_EINVAL, "negative size");
+  if (nbands > kBriggsMaxBands)
+    return fail(DDF_EINVAL, "ddf_weights_briggs_noalloc handles at most 64 bands (use ddf_weights_briggs)");
+  if (nbands == 0 || npix == 0) return DDF_OK;
+  if (!grid) return fail(DDF_EINVAL, "null pointer");
+  double* part = nullptr;
+  int rc = scratch_address(&part);
+  if (rc) return rc;
+  // every partial the second launch reads is written by the first in this call: nothing to clear
+  const double num = 5.0 * std::pow(10.0, -robust);
+  const double num2 = num * num;
+  const dim3 grid2(kBriggsBlocks, unsigned(nbands));
+  hipLaunchKernelGGL(weights_briggs_sums_k, grid2, dim3(256), 0, static_cast<hipStream_t>(stream), grid, npix, part);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(weights_briggs_apply_k, grid2, dim3(256), 0, static_cast<hipStream_t>(stream), grid, npix, part,
                      num2);
   HIP_TRY(hipGetLastError());
   return DDF_OK;

@@ -188,6 +188,12 @@ int ddf_weights_accumulate(double* grid, int64_t grid_len, const float* weights,
 int64_t ddf_weights_briggs_work_size(int nbands);
 int ddf_weights_briggs(double* grid, int nbands, int64_t npix, double robust, double* work, int64_t work_len,
                        void* stream);
+/* ddf_weights_briggs without the workspace argument, allocating nothing: the partial sums live in
+ * a library-owned device array (256 KB per GPU, held until the process ends), so nbands <= 64.
+ * No counterpart in the reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops).  Calls that overlap on one GPU share the array and race; results equal
+ * ddf_weights_briggs's. */
+int ddf_weights_briggs_noalloc(double* grid, int nbands, int64_t npix, double robust, void* stream);
 
 /* Replaces the renormalisation of ClassVisServer._finalizeWeights_handler
  * (ClassVisServer.py:1035-1040): weights[i] /= grid[index[i]] with indices >= grid_len set to 0
