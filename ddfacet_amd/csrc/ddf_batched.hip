@@ -780,53 +780,33 @@ __device__ __forceinline__ float lane_f32(float x, int j) { return __int_as_floa
 // io * OverS + jo (used with the transposed CF planes) and cfoff the element offset of the record's
 // oversampled S x S sub-kernel in its facet's CF buffer.  Cell c of the stamp (row c / S, column
 // c % S) sits at tile offset base + (c / S) W + c % S and takes CF value cf[cfoff + c].
-struct Stamp {
-  const DDF_GLOBAL float2* cfp;
-  int S, S2, base;
-  float invS;
+__device__ __forceinline__ int stamp_support(uint64_t srec) { return int((srec >> 16) & 0x1F); }
+__device__ __forceinline__ int stamp_base(uint64_t srec) { return int(srec & 0xFFFF); }
+
+// Per-lane cell layout of a stamp of support S for the first 256 cells (cell c = 64 r + lane):
+// off[r] its tile offset from the stamp's first cell (-1 past the stamp), idx8[r] the byte offset
+// of the CF value it reads (cell 0 past the stamp, so that those loads stay inside the sub-kernel).
+// It depends on S only, and the records of a work item are sorted by CF plane, so a pass recomputes
+// it only when S changes.  (c + 0.5) / S is at least 0.5 / S from an integer, far above the error
+// of (c + 0.5) * rcp(S) in FP32 for c < 1024, S < 32 (below 1e-3), so the row is exact.
+template <int G>
+struct StampLayout {
+  int S;
+  int off[G];
+  uint32_t idx8[G];
 };
-__device__ __forceinline__ Stamp stamp_none() {
-  Stamp s;
-  s.cfp = nullptr;
-  s.S = s.S2 = s.base = 0;
-  s.invS = 0.f;
-  return s;
-}
-__device__ __forceinline__ Stamp stamp_of(uint64_t srec, const DDF_GLOBAL float2* cf) {
-  Stamp s;
-  s.base = int(srec & 0xFFFF);
-  s.S = int((srec >> 16) & 0x1F);
-  s.S2 = s.S * s.S;
-  s.cfp = cf + uint32_t(srec >> 32);
-  // v_rcp_f32 (1 ulp): ample for stamp_cell below
-  s.invS = __builtin_amdgcn_rcpf(float(s.S));
-  return s;
-}
-// Tile offset of cell c, or -1 past the stamp.  (c + 0.5) / S is at least 0.5 / S from an integer,
-// far above the error of (c + 0.5) * rcp(S) in FP32 for any stamp that fits a tile (c < 1024,
-// S < 32: below 1e-3), so the row is exact.
-__device__ __forceinline__ int stamp_cell(const Stamp& s, int c, int W) {
-  const int sy = int((float(c) + 0.5f) * s.invS);
-  return c < s.S2 ? sy * W + (c - sy * s.S) : -1;
-}
-// The first 256 cells of a stamp (the whole stamp for S <= 15): per lane 4 cells, CF values
-// loaded unconditionally (lanes past the stamp read cell 0) so that the loads of the next record
-// can be issued before the LDS work of the current one.
-struct StampCells {
-  float2 w[4];
-  int off[4];
-};
-// Groups of 64 cells wholly past the stamp (a uniform test) are neither addressed nor loaded.
-__device__ __forceinline__ void fetch_cells(const Stamp& s, int lane, int W, StampCells& sc) {
+template <int G>
+__device__ __forceinline__ void stamp_layout(int S, int lane, int W, StampLayout<G>& L) {
+  L.S = S;
+  const int S2 = S * S;
+  const float invS = __builtin_amdgcn_rcpf(float(S));
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    sc.off[r] = -1;
-    sc.w[r] = make_float2(0.f, 0.f);
-    if (64 * r < s.S2) {
-      const int c = 64 * r + lane;
-      sc.off[r] = stamp_cell(s, c, W);
-      sc.w[r] = ld2(s.cfp + (c < s.S2 ? c : 0));
-    }
+  for (int r = 0; r < G; ++r) {
+    const int c = 64 * r + lane;
+    const int sy = int((float(c) + 0.5f) * invS);
+    const bool in = c < S2;
+    L.off[r] = in ? sy * W + (c - sy * S) : -1;
+    L.idx8[r] = in ? uint32_t(c) * uint32_t(sizeof(float2)) : 0u;
   }
 }
 
@@ -834,38 +814,109 @@ __device__ __forceinline__ void fetch_cells(const Stamp& s, int lane, int W, Sta
 // One single-wavefront workgroup per bin item, with a private LDS tile image (T + 2 h)^2 per
 // polarisation, (re, im) interleaved.  The wavefront loads 64 records (stamp record, value) at a
 // time, then adds their stamps one after the other with the record in scalar registers; lanes are
-// stamp cells (4 per lane) and the CF values of the next record are loaded while the current one
-// is added.  The adds are plain LDS read-modify-writes: the cells of one stamp are distinct lanes'
-// and a wavefront's LDS operations complete in issue order, so no atomics are needed (LDS float
-// atomics serialise over the lanes).  The tile then goes to its slot (or, without slots, into the
-// facet grid with global float atomics on the non-zero cells).
-template <bool CONJ>
-__device__ __forceinline__ void stamp_add(float2* tile, int WW, int npg, const Stamp& s, const StampCells& sc,
-                                          const float* vr, const float* vi) {
+// stamp cells, G groups of 64 cells per record, with G the largest ceil(S^2 / 64) of the 64
+// records (records are sorted by CF plane, so mostly that of every record).  Every record issues
+// exactly G CF loads, so the number of loads in flight is known at compile time and the wait before
+// a record's adds covers its own loads only: the CF values of the records two ahead are loaded
+// while the current one is added.  Addresses: the record's sub-kernel and tile position are
+// scalars, the lane's cell offsets come from the support's StampLayout (kept across records of one
+// support), so a cell costs an address add and the complex multiply-add.  The adds are plain LDS
+// read-modify-writes: the cells of one stamp are distinct lanes' and a wavefront's LDS operations
+// complete in issue order, so no atomics are needed.  The tile then goes to its slot (or, without
+// slots, into the facet grid with global float atomics on the non-zero cells).
+template <bool CONJ, bool ONE, bool BIG, int G>
+__device__ __forceinline__ void grid_b_batch(float2* tile, int W, int WW, int npg, int lane, const DDF_GLOBAL char* cf,
+                                             uint64_t my_rec, const float2 (&my_v)[4], int cnt) {
+  StampLayout<G> Lload, Ladd;
+  Lload.S = Ladd.S = -1;
+  uint64_t rA = 0, rB = 0, rC = 0;
+  float2 vA[G], vB[G], vC[G];
+  // records past the batch load the last one again (their values are not used)
+  auto load = [&](int j, uint64_t& rec, float2 (&v)[G]) {
+    rec = lane_u64(my_rec, j < cnt ? j : cnt - 1);
+    const int S = stamp_support(rec);
+    if (S != Lload.S) stamp_layout<G>(S, lane, W, Lload);
+    const DDF_GLOBAL char* cfp = cf + (size_t(uint32_t(rec >> 32)) << 3);
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    if (64 * r >= s.S2) break;
-    if (sc.off[r] < 0) continue;
-    const float wx = sc.w[r].x, wy = CONJ ? -sc.w[r].y : sc.w[r].y;
-    float2* t = tile + s.base + sc.off[r];
+    for (int r = 0; r < G; ++r) v[r] = ld2((const DDF_GLOBAL float2*)(cfp + Lload.idx8[r]));
+  };
+  auto add = [&](int j, uint64_t rec, const float2 (&v)[G]) {
+    if (j >= cnt) return;
+    float vr[4], vi[4];
 #pragma unroll
     for (int ip = 0; ip < 4; ++ip) {
       if (ip < npg) {
-        float2 x = t[ip * WW];
-        x.x += vr[ip] * wx - vi[ip] * wy;
-        x.y += vr[ip] * wy + vi[ip] * wx;
-        t[ip * WW] = x;
+        vr[ip] = lane_f32(my_v[ip].x, j);
+        vi[ip] = lane_f32(my_v[ip].y, j);
       }
     }
+    const int S = stamp_support(rec), base = stamp_base(rec);
+    if (S != Ladd.S) stamp_layout<G>(S, lane, W, Ladd);
+    const int S2 = S * S;
+    float2* tb = tile + base;
+#pragma unroll
+    for (int r = 0; r < G; ++r) {
+      if (64 * r >= S2) break;
+      if (Ladd.off[r] >= 0) {
+        const float wx = v[r].x, wy = CONJ ? -v[r].y : v[r].y;
+        float2* t = tb + Ladd.off[r];
+#pragma unroll
+        for (int ip = 0; ip < 4; ++ip) {
+          if (ip < npg) {
+            float2 x = t[ip * WW];
+            x.x = fmaf(vr[ip], wx, x.x);
+            x.x = fmaf(-vi[ip], wy, x.x);
+            x.y = fmaf(vr[ip], wy, x.y);
+            x.y = fmaf(vi[ip], wx, x.y);
+            t[ip * WW] = x;
+          }
+        }
+      }
+    }
+    // stamps wider than 16 x 16: the cells past the first 256, without prefetch
+    if (BIG && S2 > 256) {
+      const DDF_GLOBAL float2* cfp = (const DDF_GLOBAL float2*)(cf + (size_t(uint32_t(rec >> 32)) << 3));
+      const float invS = __builtin_amdgcn_rcpf(float(S));
+      for (int c = 256 + lane; c < S2; c += 64) {
+        const int sy = int((float(c) + 0.5f) * invS);
+        const float2 w = ld2(cfp + c);
+        const float wx = w.x, wy = CONJ ? -w.y : w.y;
+        float2* t = tb + sy * W + (c - sy * S);
+#pragma unroll
+        for (int ip = 0; ip < 4; ++ip) {
+          if (ip < npg) {
+            float2 x = t[ip * WW];
+            x.x = fmaf(vr[ip], wx, x.x);
+            x.x = fmaf(-vi[ip], wy, x.x);
+            x.y = fmaf(vr[ip], wy, x.y);
+            x.y = fmaf(vi[ip], wx, x.y);
+            t[ip * WW] = x;
+          }
+        }
+      }
+    }
+  };
+  // Three records in flight, in statically rotating buffers (A, B, C), so no register copy has to
+  // wait for a load.
+  load(0, rA, vA);
+  load(1, rB, vB);
+  for (int j = 0; j < cnt; j += 3) {
+    load(j + 2, rC, vC);
+    add(j, rA, vA);
+    load(j + 3, rA, vA);
+    add(j + 1, rB, vB);
+    load(j + 4, rB, vB);
+    add(j + 2, rC, vC);
   }
 }
 
 // ONE: a single polarisation (npg == 1), with the per-polarisation loops resolved at compile time.
-template <bool CONJ, bool ONE>
+// BIG: stamps may have more than 256 cells (halo > 7).
+template <bool CONJ, bool ONE, bool BIG>
 __global__ __launch_bounds__(64) void grid_pass_b(BatchParams p, const uint64_t* bstamp, const float2* V,
                                                   const BinItem* items, const uint32_t* recs, uint32_t* queue,
                                                   float2* slots) {
-  // 16-byte aligned: the float2 read-modify-writes below must not straddle 8-byte boundaries
+  // 16-byte aligned: the float2 read-modify-writes must not straddle 8-byte boundaries
   extern __shared__ __attribute__((aligned(16))) float lds_f[];
   float2* tile = reinterpret_cast<float2*>(lds_f);  // [npg][W * W]
   __shared__ uint32_t s_item;
@@ -876,11 +927,11 @@ __global__ __launch_bounds__(64) void grid_pass_b(BatchParams p, const uint64_t*
   while (true) {
     if (lane == 0) s_item = atomicAdd(queue, 1u);
     __syncthreads();
-    const uint32_t it = s_item;
+    const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
     if (it >= p.nitems_b) break;
     const BinItem bi = items[it];
     const FacetDev fd = gload(p.facets + bi.facet);
-    const DDF_GLOBAL float2* cf = gptr(fd.cf);
+    const DDF_GLOBAL char* cf = gptr(reinterpret_cast<const char*>(fd.cf));
     for (int i = lane; i < npg * WW; i += 64) tile[i] = make_float2(0.f, 0.f);
     const int x0 = bi.tx * p.tile - p.halo, y0 = bi.ty * p.tile - p.halo;
     for (uint32_t k0 = bi.begin; k0 < bi.end; k0 += 64u) {
@@ -894,47 +945,15 @@ __global__ __launch_bounds__(64) void grid_pass_b(BatchParams p, const uint64_t*
 #pragma unroll
       for (int ip = 0; ip < 4; ++ip) my_v[ip] = (ok && ip < npg) ? V[my_v0 + ip] : make_float2(0.f, 0.f);
       const int cnt = int(min(64u, bi.end - k0));
-      // Three records in flight: the CF values of record j + 2 are loaded while records j and
-      // j + 1 are added.  The buffers rotate statically (A, B, C), so no register copy has to
-      // wait for a load.
-      Stamp sA = stamp_none(), sB = stamp_none(), sC = stamp_none();
-      StampCells cA, cB, cC;
-      auto load = [&](int j, Stamp& st_, StampCells& c) {
-        if (j < cnt) {
-          st_ = stamp_of(lane_u64(my_rec, j), cf);
-          fetch_cells(st_, lane, W, c);
-        }
-      };
-      auto add = [&](int j, const Stamp& st_, const StampCells& c) {
-        if (j >= cnt) return;
-        float vr[4], vi[4];
-#pragma unroll
-        for (int ip = 0; ip < 4; ++ip) {
-          vr[ip] = lane_f32(my_v[ip].x, j);
-          vi[ip] = lane_f32(my_v[ip].y, j);
-        }
-        stamp_add<CONJ>(tile, WW, npg, st_, c, vr, vi);
-        // stamps wider than 16 x 16: the cells past the first 256, without prefetch
-        for (int c0 = 256; c0 < st_.S2; c0 += 256) {
-          StampCells xc;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int c1 = c0 + 64 * r + lane;
-            xc.off[r] = stamp_cell(st_, c1, W);
-            xc.w[r] = ld2(st_.cfp + (c1 < st_.S2 ? c1 : 0));
-          }
-          stamp_add<CONJ>(tile, WW, npg, st_, xc, vr, vi);
-        }
-      };
-      load(0, sA, cA);
-      load(1, sB, cB);
-      for (int j = 0; j < cnt; j += 3) {
-        load(j + 2, sC, cC);
-        add(j, sA, cA);
-        load(j + 3, sA, cA);
-        add(j + 1, sB, cB);
-        load(j + 4, sB, cB);
-        add(j + 2, sC, cC);
+      // G = the largest ceil(S^2 / 64) of the batch, capped at 4
+      const int myS = stamp_support(my_rec);
+      const int myG = (myS * myS + 63) >> 6;
+      const int G = __ballot(myG > 3) ? 4 : __ballot(myG > 2) ? 3 : __ballot(myG > 1) ? 2 : 1;
+      switch (G) {
+        case 1: grid_b_batch<CONJ, ONE, BIG, 1>(tile, W, WW, npg, lane, cf, my_rec, my_v, cnt); break;
+        case 2: grid_b_batch<CONJ, ONE, BIG, 2>(tile, W, WW, npg, lane, cf, my_rec, my_v, cnt); break;
+        case 3: grid_b_batch<CONJ, ONE, BIG, 3>(tile, W, WW, npg, lane, cf, my_rec, my_v, cnt); break;
+        default: grid_b_batch<CONJ, ONE, BIG, 4>(tile, W, WW, npg, lane, cf, my_rec, my_v, cnt); break;
       }
     }
     if (slots) {
@@ -1542,12 +1561,23 @@ static unsigned persistent_wgs(K kernel, size_t lds, unsigned nwg, int block = 2
 
 // Persistent grid sizes of the tile passes (grid B, degrid A, adjoint of degrid A), computed once
 // per plan so that the launches make no device queries (they may run under stream capture).
+// grid_pass_b instance for a tile pass: CONJ, one polarisation or several, stamps beyond 256 cells.
+typedef void (*GridPassB)(BatchParams, const uint64_t*, const float2*, const BinItem*, const uint32_t*, uint32_t*,
+                          float2*);
+template <bool CONJ>
+static GridPassB grid_pass_b_for(int npg, int halo) {
+  const bool big = halo > 7;
+  if (npg == 1) return big ? grid_pass_b<CONJ, true, true> : grid_pass_b<CONJ, true, false>;
+  return big ? grid_pass_b<CONJ, false, true> : grid_pass_b<CONJ, false, false>;
+}
+
+// Persistent grid sizes of the tile passes (grid B, degrid A, adjoint of degrid A), computed once
+// per plan so that the launches make no device queries (they may run under stream capture).
 int batched_tile_wgs(const BatchParams& pg, const BatchParams& pd, unsigned out[3]) {
   const int Wg = pg.tile + 2 * pg.halo, Wd = pd.tile + 2 * pd.halo;
   const int npg = pg.nstokes < pg.npol ? pg.nstokes : pg.npol;
   const int npd = pd.nstokes < pd.npol ? pd.nstokes : pd.npol;
-  out[0] = npg == 1 ? persistent_wgs(grid_pass_b<false, true>, size_t(npg) * Wg * Wg * sizeof(float2), 0, 64)
-                    : persistent_wgs(grid_pass_b<false, false>, size_t(npg) * Wg * Wg * sizeof(float2), 0, 64);
+  out[0] = persistent_wgs(grid_pass_b_for<false>(npg, pg.halo), size_t(npg) * Wg * Wg * sizeof(float2), 0, 64);
   const size_t lds_d = size_t(pd.nstokes) * Wd * Wd * 2 * sizeof(float);
   if (pd.nstokes == 1)
     out[1] = pd.prec != 2 ? persistent_wgs(degrid_pass_a<double, true>, lds_d, 0)
@@ -1555,8 +1585,7 @@ int batched_tile_wgs(const BatchParams& pg, const BatchParams& pd, unsigned out[
   else
     out[1] = pd.prec != 2 ? persistent_wgs(degrid_pass_a<double, false>, lds_d, 0)
                           : persistent_wgs(degrid_pass_a<float, false>, lds_d, 0);
-  out[2] = npd == 1 ? persistent_wgs(grid_pass_b<true, true>, size_t(pd.nstokes) * Wd * Wd * sizeof(float2), 0, 64)
-                    : persistent_wgs(grid_pass_b<true, false>, size_t(pd.nstokes) * Wd * Wd * sizeof(float2), 0, 64);
+  out[2] = persistent_wgs(grid_pass_b_for<true>(npd, pd.halo), size_t(pd.nstokes) * Wd * Wd * sizeof(float2), 0, 64);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
@@ -1566,12 +1595,8 @@ int batched_grid_b(const BatchParams& p, const uint64_t* place, const float2* V,
   const int W = p.tile + 2 * p.halo;
   const int npg = p.nstokes < p.npol ? p.nstokes : p.npol;
   size_t lds = size_t(npg) * W * W * sizeof(float2);
-  if (npg == 1)
-    hipLaunchKernelGGL((grid_pass_b<false, true>), dim3(nwg), dim3(64), lds, st, p, place, V, items, recs, queue,
-                       reinterpret_cast<float2*>(slots));
-  else
-    hipLaunchKernelGGL((grid_pass_b<false, false>), dim3(nwg), dim3(64), lds, st, p, place, V, items, recs, queue,
-                       reinterpret_cast<float2*>(slots));
+  hipLaunchKernelGGL(grid_pass_b_for<false>(npg, p.halo), dim3(nwg), dim3(64), lds, st, p, place, V, items, recs,
+                     queue, reinterpret_cast<float2*>(slots));
   if (hipGetLastError() != hipSuccess) return DDF_EHIP;
   if (slots) {
     const int64_t total = int64_t(p.nf) * p.nband * npg * p.ny * p.nx;
@@ -1592,12 +1617,8 @@ int batched_degrid_a_adjoint(const BatchParams& p, const uint64_t* place, const 
   BatchParams q = p;
   q.npol = p.npol;
   const int npg = q.nstokes < q.npol ? q.nstokes : q.npol;
-  if (npg == 1)
-    hipLaunchKernelGGL((grid_pass_b<true, true>), dim3(nwg), dim3(64), lds, st, q, place, Mbar, items, recs, queue,
-                       static_cast<float2*>(nullptr));
-  else
-    hipLaunchKernelGGL((grid_pass_b<true, false>), dim3(nwg), dim3(64), lds, st, q, place, Mbar, items, recs, queue,
-                       static_cast<float2*>(nullptr));
+  hipLaunchKernelGGL(grid_pass_b_for<true>(npg, q.halo), dim3(nwg), dim3(64), lds, st, q, place, Mbar, items, recs,
+                     queue, static_cast<float2*>(nullptr));
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
