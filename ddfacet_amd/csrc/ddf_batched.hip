@@ -383,6 +383,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
   // is skipped, i.e. it opens a run of one key.
   constexpr int kW = 0, kF = 1, kRow = 2, kCh = 3, kV = 4;
   constexpr int kKey = grid_a_key_offset(READ), kU = kKey + 6, kStride = grid_a_stride(READ);
+  // mixed precision (except with ScaleJones, which needs the uvw in metres): the staged uvw is
+  // scaled by kPhaseScale nu / 2 pi, so that the phase of a (visibility, facet) pair in
+  // revolutions is uvw' . (l0, m0, n0), three FMAs with no per-row state
+  constexpr bool kScaled = PREC == 0 && JMODE != 3;
   // 16-byte aligned staging buffers
   float* s_stage = reinterpret_cast<float*>(lds + ((p.nf * per_f + nchan_acc + 1) & ~1));
   for (int i = threadIdx.x; i < p.nf * per_f + nchan_acc; i += blockDim.x) lds[i] = 0.;
@@ -485,6 +489,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
         BBblk = jblk.BB;
       }
       double* sch = s_chan + lf * cs;
+      double* sch1 = sch + v.nchan;  // second half: the weight sums
       using RT = typename std::conditional<PREC == 1, double, float>::type;
       // Per-visibility Jones without ScaleJones (JMODE 1) depends only on the key (baseline,
       // Jones time, Jones channel), so J0^H (sum_v meas_v w_v) J1 over a run of one key equals
@@ -527,6 +532,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
       // row state of the walk
       int64_t row = -1;
       double uvwr[3] = {0, 0, 0};
+      // phasors: with the scaled uvw through phasor_rev, otherwise ChanPhasor along each row
       ChanPhasor<PREC> ph;
       const int nch = br.ch1 - br.ch0;
       const int nvis = br.nr * nch;
@@ -550,6 +556,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
           float* e = stg + i * kStride;
           if (srow < 0 || srow >= v.nrow) {
             e[kF] = __int_as_float(1);
+            if (kScaled) {
+              // walked without a skip test: a zero entry on a valid channel
+              e[kW] = 0.f;
+#pragma unroll
+              for (int k = 0; k < 2 * READ; ++k) e[kV + k] = 0.f;
+              e[kCh] = __int_as_float(cc);
+              double* eu = reinterpret_cast<double*>(e + kU);
+              eu[0] = eu[1] = eu[2] = 0.;
+            }
             continue;
           }
           const size_t vc = size_t(srow) * v.nchan + cc;
@@ -565,9 +580,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
           e[kRow] = __int_as_float(int(srow));
           e[kCh] = __int_as_float(cc);
           double* eu = reinterpret_cast<double*>(e + kU);
-          eu[0] = q[0];
-          eu[1] = q[1];
-          eu[2] = q[2];
+          if (kScaled) {
+            const double sc = gptr(v.freqs)[cc] * (kPhaseScale * kInvTwoPi);
+            eu[0] = q[0] * sc;
+            eu[1] = q[1] * sc;
+            eu[2] = q[2] * sc;
+          } else {
+            eu[0] = q[0];
+            eu[1] = q[1];
+            eu[2] = q[2];
+          }
           if (JMODE == 1) {
             int* ek = reinterpret_cast<int*>(e + kKey);
             int k[6];
@@ -587,9 +609,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
                 for (int j = 0; j < 6; ++j) opens = opens || kp[j] != k[j];
               }
             }
-            if (opens) fl |= 2;
+            if (opens && !(kScaled && (fl & 1))) fl |= 2;
           }
           e[kF] = __int_as_float(fl);
+          if (kScaled && (fl & 1)) {
+            // a flagged visibility is walked as a zero entry (no skip test, no run of its own)
+            e[kW] = 0.f;
+#pragma unroll
+            for (int k = 0; k < 2 * READ; ++k) e[kV + k] = 0.f;
+          }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -611,17 +639,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
               const float2 a = *reinterpret_cast<const float2*>(e + kV);
               ev[0] = a.x; ev[1] = a.y;
             }
-            const int fl = __float_as_int(hd.y);
-            if (fl & 1) continue;
-            const int cv_ = __float_as_int(hd.w);
-            const int erow = __float_as_int(hd.z);
-            if (erow != row) {
-              row = erow;
+            double su[3] = {0, 0, 0};
+            if (kScaled) {
               const double* eu = reinterpret_cast<const double*>(e + kU);
-              uvwr[0] = eu[0];
-              uvwr[1] = eu[1];
-              uvwr[2] = eu[2];
-              ph.row(kPhaseScale * (uvwr[0] * fd.l0 + uvwr[1] * fd.m0 + uvwr[2] * fd.n0));
+              su[0] = eu[0];
+              su[1] = eu[1];
+              su[2] = eu[2];
+            }
+            const int fl = __float_as_int(hd.y);
+            // with the scaled uvw, skipped visibilities are staged as zero entries and walked like
+            // the others (their weight, values and run bit are zero)
+            if (!kScaled && (fl & 1)) continue;
+            const int cv_ = __float_as_int(hd.w);
+            if (!kScaled) {
+              const int erow = __float_as_int(hd.z);
+              if (erow != row) {
+                row = erow;
+                const double* eu = reinterpret_cast<const double*>(e + kU);
+                uvwr[0] = eu[0];
+                uvwr[1] = eu[1];
+                uvwr[2] = eu[2];
+                ph.row(kPhaseScale * (uvwr[0] * fd.l0 + uvwr[1] * fd.m0 + uvwr[2] * fd.n0));
+              }
             }
             const double fw = double(hd.x);
             const double fwd = fw * dec2;
@@ -639,7 +678,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
               }
               rfw += fw;
               if (!p.dopsf) {
-                const cx<RT> wt = ph.template weighted<RT>(v, cv_, hd.x);
+                cx<RT> wt;
+                if (kScaled) {
+                  const float2 pz = phasor_rev(su[0] * fd.l0 + su[1] * fd.m0 + su[2] * fd.n0);
+                  wt = cx<RT>{RT(pz.x * hd.x), RT(pz.y * hd.x)};
+                } else {
+                  wt = ph.template weighted<RT>(v, cv_, hd.x);
+                }
                 if (READ == 4) {
 #pragma unroll
                   for (int q = 0; q < 4; ++q) racc[q] = racc[q] + cx<RT>{RT(ev[2 * q]), RT(ev[2 * q + 1])} * wt;
@@ -655,7 +700,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
               const double x1 = ndec2 * fw, x0 = cBB * x1;
               if (p.chan_lds) {
                 atomicAdd(&sch[cv_], x0);
-                atomicAdd(&sch[v.nchan + cv_], x1);
+                atomicAdd(&sch1[cv_], x1);
               } else {
                 atomic_add_g(gptr_w(fd.sum_jones_chan) + cv_, x0);
                 atomic_add_g(gptr_w(fd.sum_jones_chan) + (v.nchan + cv_), x1);
@@ -673,7 +718,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
                 meas[0] = cd{ev[0], ev[1]}; meas[1] = cd{0, 0}; meas[2] = cd{0, 0}; meas[3] = meas[0];
               }
               cd wt = cd{fw, 0};
-              if (!p.dopsf) wt = ph.at(v, cv_) * fw;
+              if (!p.dopsf) {
+                if (kScaled) {
+                  const float2 pz = phasor_rev(su[0] * fd.l0 + su[1] * fd.m0 + su[2] * fd.n0);
+                  wt = cd{double(pz.x), double(pz.y)} * fw;
+                } else {
+                  wt = ph.at(v, cv_) * fw;
+                }
+              }
               if (JMODE == 3) {
                 const JonesPair& jj = jc.get(jp, row, cv_, uvwr, p.wl_mean);
                 M2 m;
@@ -698,7 +750,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
               }
               if (JMODE != 0) {
                 const double x = double(npg) * fwd;
-                if (p.chan_lds) atomicAdd(&sch[v.nchan + cv_], x);
+                if (p.chan_lds) atomicAdd(&sch1[cv_], x);
                 else atomic_add_g(gptr_w(fd.sum_jones_chan) + (v.nchan + cv_), x);
               }
               if (READ == 4) {
@@ -1202,10 +1254,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
       fr = gptr(v.freqs)[c];
     }
     if (PREC == 0) {
-      // mixed precision: uvw pre-scaled by -kPhaseScale freq, so that the phase of facet f is
-      // u0 l0 + u1 m0 + u2 n0 (three FP64 operations per facet; the rounding differs from forming
-      // the angle first by a few ulp of the phase)
-      const double sc = -kPhaseScale * fr;
+      // mixed precision: uvw pre-scaled by -kPhaseScale freq / 2 pi, so that the phase of facet f
+      // in revolutions is u0 l0 + u1 m0 + u2 n0 (three FP64 operations per facet; the rounding
+      // differs from forming the angle first by a few ulp of the phase), the phasor phasor_rev of
+      // it, as in the gridding pass A (the adjoint)
+      const double sc = -kPhaseScale * kInvTwoPi * fr;
       sU[0][t] = u0 * sc;
       sU[1][t] = u1 * sc;
       sU[2][t] = u2 * sc;
@@ -1291,7 +1344,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
         for (int fi = 0; fi < FB; ++fi) {
           if (fi >= nfb || !((nzw >> (8 * fi)) & 0xFFu)) continue;
           const double dot = u0 * sGeo[fi][0] + u1 * sGeo[fi][1] + u2 * sGeo[fi][2];
-          const cx<RT> corr = cvt<RT>(PREC == 0 ? phasor(dot) : phasor_prec<PREC>(fr, -kPhaseScale * dot));
+          cx<RT> corr;
+          if (PREC == 0) {
+            const float2 pz = phasor_rev(dot);
+            corr = cx<RT>{RT(pz.x), RT(pz.y)};
+          } else {
+            corr = cvt<RT>(phasor_prec<PREC>(fr, -kPhaseScale * dot));
+          }
 #pragma unroll
           for (int q = 0; q < 4; ++q) acc[q] = acc[q] + cvt<double>(sY[fi][myrun][q] * corr);
         }

@@ -204,8 +204,9 @@ struct BatchParams {
 };
 
 // Staged visibility entry of gridding pass A (floats): weight, flag word, row, channel; READ complex
-// values from offset 4; the Jones key (6 ints) from grid_a_key_offset; uvw (3 doubles) after it;
-// the stride a multiple of 4 floats so that every entry is 16-byte aligned.
+// values from offset 4; the Jones key (6 ints) from grid_a_key_offset; uvw (3 doubles) after it
+// (in the mixed precision scaled to revolutions per unit direction cosine); the stride a multiple
+// of 4 floats so that every entry is 16-byte aligned.
 constexpr int grid_a_key_offset(int read) { return (4 + 2 * read + 3) & ~3; }
 constexpr int grid_a_stride(int read) { return (grid_a_key_offset(read) + 12 + 3) & ~3; }
 
