@@ -1182,15 +1182,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
                                                      const uint64_t* vrc, const uint32_t* vblk,
                                                      const uint32_t* vrun, const uint32_t* run_first, int64_t nvl) {
   using RT = typename std::conditional<PREC == 1, double, float>::type;
-  constexpr int FB = 4;
-  __shared__ cx<RT> sY[FB][256][4];
-  // per run, one byte per facet of the batch (read as one word): the run's model values are not
-  // all zero
-  __shared__ __attribute__((aligned(4))) uint8_t sNZ[256][FB];
+  // Y of (run, facet) for a batch of facets, [facet][run][correlation]: as many facets per batch
+  // as fit for the tile's run count (a tile of 256 visibilities holds ~20 runs with BDA, so 2
+  // batches cover 81 facets; 4 facets when every visibility is a run of its own)
+  constexpr int kYCap = 4 * 256 * 4;
+  constexpr int kFMax = 64;
+  __shared__ cx<RT> sY[kYCap];
   __shared__ RunInfo sRun[256];
   // l0, m0, n0 of the batch's facets: the visibility loop reads them from LDS, not from the facet
   // table in global memory (whose loads the compiler cannot hoist or move to the scalar unit)
-  __shared__ double sGeo[FB][3];
+  __shared__ double sGeo[kFMax][3];
   // mixed precision: each lane's pre-scaled uvw, read back after every batch's barrier so that it
   // is not held in registers (or spilled) across the forming of Y
   __shared__ double sU[3][256];
@@ -1263,29 +1264,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
       sU[1][t] = u1 * sc;
       sU[2][t] = u2 * sc;
     }
+    // the sum over facets, in FP64
     cd acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     __syncthreads();
-    for (int f0 = 0; f0 < p.nf; f0 += FB) {
-      const int nfb = min(FB, p.nf - f0);
+    const int FBd = min(min(p.nf, kFMax), max(1, kYCap / (4 * nr)));
+    for (int f0 = 0; f0 < p.nf; f0 += FBd) {
+      const int nfb = min(FBd, p.nf - f0);
       if (t < nfb) {
         const FacetDev fd = gload(p.facets + f0 + t);
         sGeo[t][0] = fd.l0;
         sGeo[t][1] = fd.m0;
         sGeo[t][2] = fd.n0;
       }
-      // Y of every (run, facet of the batch)
+      // Y of every (run, facet of the batch); facets vary fastest over the threads, so the Jones
+      // loads of one run's facets (directions) are contiguous in the prepared tables
       for (int j = t; j < nr * nfb; j += 256) {
-        const int fi = j / nr;
-        const int ir = j - fi * nr;
+        const int ir = j / nfb;
+        const int fi = j - ir * nfb;
         const int f = f0 + fi;
         const RunInfo ri = sRun[ir];
-        int nz = 0;
         M2T<RT> cv;
 #pragma unroll
         for (int q = 0; q < 4; ++q) cv.a[q] = cx<RT>{0, 0};
         if (ri.row >= 0) {
           const DDF_GLOBAL float2* m = gptr(M) + (size_t(f) * v.nblocks + ri.blk) * p.nstokes;
           cd st[4];
+          int nz = 0;
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             if (q < p.nstokes) {
@@ -1328,31 +1332,56 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
             }
           }
         }
+        cx<RT>* yo = sY + (size_t(fi) * nr + ir) * 4;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) sY[fi][ir][q] = cv.a[q];
-        sNZ[ir][fi] = uint8_t(nz);
+        for (int q = 0; q < 4; ++q) yo[q] = cv.a[q];
       }
       __syncthreads();
       if (live) {
-        const uint32_t nzw = *reinterpret_cast<const uint32_t*>(sNZ[myrun]);
         if (PREC == 0) {
           u0 = sU[0][t];
           u1 = sU[1][t];
           u2 = sU[2][t];
         }
+        // facets whose Y is zero for this run (an empty model there) are skipped; in the mixed
+        // precision groups of 8 facets are summed by FP32 fused multiply-adds first, then added
+        // to the FP64 sum
+        for (int g0 = 0; g0 < nfb; g0 += 8) {
+          const int g1 = min(g0 + 8, nfb);
+          cf bacc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+          for (int fi = g0; fi < g1; ++fi) {
+            const cx<RT>* yp = sY + (size_t(fi) * nr + myrun) * 4;
+            cx<RT> y[4];
 #pragma unroll
-        for (int fi = 0; fi < FB; ++fi) {
-          if (fi >= nfb || !((nzw >> (8 * fi)) & 0xFFu)) continue;
-          const double dot = u0 * sGeo[fi][0] + u1 * sGeo[fi][1] + u2 * sGeo[fi][2];
-          cx<RT> corr;
-          if (PREC == 0) {
-            const float2 pz = phasor_rev(dot);
-            corr = cx<RT>{RT(pz.x), RT(pz.y)};
-          } else {
-            corr = cvt<RT>(phasor_prec<PREC>(fr, -kPhaseScale * dot));
+            for (int q = 0; q < 4; ++q) y[q] = yp[q];
+            bool nz = false;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) nz = nz || y[q].re != RT(0) || y[q].im != RT(0);
+            if (!nz) continue;
+            const double dot = u0 * sGeo[fi][0] + u1 * sGeo[fi][1] + u2 * sGeo[fi][2];
+            cx<RT> corr;
+            if (PREC == 0) {
+              const float2 pz = phasor_rev(dot);
+              corr = cx<RT>{RT(pz.x), RT(pz.y)};
+            } else {
+              corr = cvt<RT>(phasor_prec<PREC>(fr, -kPhaseScale * dot));
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              if (PREC == 0) {
+                bacc[q].re = fmaf(float(y[q].re), float(corr.re), bacc[q].re);
+                bacc[q].re = fmaf(-float(y[q].im), float(corr.im), bacc[q].re);
+                bacc[q].im = fmaf(float(y[q].re), float(corr.im), bacc[q].im);
+                bacc[q].im = fmaf(float(y[q].im), float(corr.re), bacc[q].im);
+              } else {
+                acc[q] = acc[q] + cvt<double>(y[q] * corr);
+              }
+            }
           }
+          if (PREC == 0) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) acc[q] = acc[q] + cvt<double>(sY[fi][myrun][q] * corr);
+            for (int q = 0; q < 4; ++q) acc[q] = acc[q] + cvt<double>(bacc[q]);
+          }
         }
       }
       __syncthreads();
