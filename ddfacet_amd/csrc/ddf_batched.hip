@@ -459,7 +459,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
       DecorrP dp = p.dp;
       dp.l0 = fd.dec_l0;
       dp.m0 = fd.dec_m0;
-      const double dec = (doit && mid_ok) ? decorr_factor(dp, v.uvw, p.fmean0, mid) : 1.;
+      const double dec = (doit && mid_ok) ? decorr_factor(dp, v.uvw, p.tscale0, mid) : 1.;
       double swt = 0, ssw = 0, sj = 0;
       // block sums: with per-visibility Jones (JMODE 1) a sum over the block's runs (about one per
       // block), in the arithmetic type of the run sums; otherwise over visibilities, in FP64
@@ -1174,7 +1174,7 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
 struct RunInfo {
   int row, chan, mid;  // representative row and channel; middle row of the block (-1: none)
   uint32_t blk;
-  double fmean;        // mean frequency of the block's channels (decorrelation)
+  double tscale;       // decorr_tscale of the mean frequency of the block's channels
 };
 
 template <int NCORR, int JMODE, int PREC>
@@ -1208,7 +1208,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
       ri.chan = 0;
       ri.mid = -1;
       ri.blk = 0;
-      ri.fmean = 0;
+      ri.tscale = 0;
       const uint32_t kr = run_first[r0 + t];
       if (int64_t(kr) < nvl) {
         const uint64_t e = vrc[kr];
@@ -1224,7 +1224,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
             double fm = 0;
             for (int cc = br.ch0; cc < br.ch1; ++cc) fm += gptr(v.freqs)[cc];
             if (br.ch1 > br.ch0) fm /= double(br.ch1 - br.ch0);
-            ri.fmean = fm;
+            ri.tscale = decorr_tscale(fm);
             if (br.nr > 0) {
               const int64_t mid = br.rows[br.nr / 2];
               if (mid >= 0 && mid < v.nrow) ri.mid = int(mid);
@@ -1326,7 +1326,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
               DecorrP dp = p.dp;
               dp.l0 = fd.dec_l0;
               dp.m0 = fd.dec_m0;
-              const RT dec = RT(decorr_factor(dp, v.uvw, ri.fmean, ri.mid));
+              const RT dec = RT(decorr_factor(dp, v.uvw, ri.tscale, ri.mid));
 #pragma unroll
               for (int q = 0; q < 4; ++q) cv.a[q] = cv.a[q] * dec;
             }
@@ -1482,7 +1482,7 @@ __global__ __launch_bounds__(256) void degrid_adj_b(BatchParams p, const VisP v,
       DecorrP dp = p.dp;
       dp.l0 = fd.dec_l0;
       dp.m0 = fd.dec_m0;
-      const double dec = mid_ok ? decorr_factor(dp, v.uvw, fm, mid) : 1.;
+      const double dec = mid_ok ? decorr_factor(dp, v.uvw, decorr_tscale(fm), mid) : 1.;
       JonesP jp = p.jp;
       jp.kdir = fd.kms_dir;
       jp.bdir = fd.beam_dir;

@@ -423,22 +423,29 @@ struct DecorrP {
   int enabled, tsmear, fsmear;
   const double* uvw_dt;
   double dt, dnu, l0, m0;
+  double fscale;  // pi dnu / c, formed on the host (decorr_fscale)
 };
+// The phase scales of the two terms, pi dnu / c and pi nu / c in the reference's operation order;
+// the caller forms them where dnu and nu are known (the host for dnu and the mean frequency of a
+// chunk, the block for a block's mean frequency), so that the passes hold no division.
+__host__ __device__ inline double decorr_fscale(double dnu) { return kPi * dnu / kC; }
+__host__ __device__ inline double decorr_tscale(double nu) { return kPi * nu / kC; }
 
-__device__ inline double decorr_factor(const DecorrP& d, const double* uvw, double nu, int64_t idx) {
+// tscale = decorr_tscale(nu) for the frequency nu of the time-smearing term.
+__device__ inline double decorr_factor(const DecorrP& d, const double* uvw, double tscale, int64_t idx) {
   if (!d.enabled) return 1.;
   double n0 = sqrt(1. - d.l0 * d.l0 - d.m0 * d.m0) - 1.;
   double f = 1.;
   if (d.fsmear) {
     const DDF_GLOBAL double* q = gptr(uvw) + 3 * idx;
     double ph = q[0] * d.l0 + q[1] * d.m0 + q[2] * n0;
-    double phi = kPi * d.dnu / kC * ph;
+    double phi = d.fscale * ph;
     f *= (phi == 0.) ? 1.0 : sin(phi) / phi;
   }
   if (d.tsmear) {
     const DDF_GLOBAL double* q = gptr(d.uvw_dt) + 3 * idx;
     double dph = (q[0] * d.l0 + q[1] * d.m0 + q[2] * n0) * d.dt;
-    double phi = kPi * nu / kC * dph;
+    double phi = tscale * dph;
     f *= (phi == 0.) ? 1.0 : sin(phi) / phi;
   }
   return f;

@@ -189,6 +189,7 @@ struct BatchParams {
   int jmode;
   DecorrP dp;
   double fmean0, wl_mean;
+  double tscale0;  // decorr_tscale(fmean0)
   int prec;
   // pass A layout
   int group, slots, ngroups;

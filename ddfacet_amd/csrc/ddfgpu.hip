@@ -155,6 +155,7 @@ struct KParams {
   int dopsf;
   double cu, cv, l0, m0, n0;
   double fmean0, wl_mean;
+  double tscale0;  // decorr_tscale(fmean0)
   CFP cf;
   JonesP jp;
   DecorrP dp;
@@ -227,7 +228,7 @@ __global__ __launch_bounds__(256) void grid_plain(KParams p) {
     if (nvis <= 0) continue;
     const int64_t mid = br.rows[br.nr / 2];
     const bool mid_ok = mid >= 0 && mid < v.nrow;
-    const double dec = mid_ok ? decorr_factor(p.dp, v.uvw, p.fmean0, mid) : 1.;
+    const double dec = mid_ok ? decorr_factor(p.dp, v.uvw, p.tscale0, mid) : 1.;
     const int gchan = v.chan_map[br.ch0];
 
     // With Jones the per-channel sums are only added for gridded blocks, so the placement is
@@ -463,7 +464,7 @@ __global__ __launch_bounds__(256) void degrid_plain(KParams p) {
       JonesPair jj = jones_at(p.jp, mid, (br.ch0 + br.ch1) / 2, v.uvw + 3 * mid, p.wl_mean);
       cv = mul(mul(jj.J0, cv), herm(jj.J1));
     }
-    const double dec = mid_ok ? decorr_factor(p.dp, v.uvw, fm, mid) : 1.;
+    const double dec = mid_ok ? decorr_factor(p.dp, v.uvw, decorr_tscale(fm), mid) : 1.;
     const int64_t nvis = int64_t(br.nr) * nch;
     for (int64_t k = lane; k < nvis; k += kWave) {
       const int r = int(k / nch);
@@ -649,6 +650,7 @@ void fill_params(KParams& p, const ddf_grid_desc* g, const ddf_vis* v, const ddf
   p.n0 = std::sqrt(1 - f->l0 * f->l0 - f->m0 * f->m0) - 1;
   p.wl_mean = wl_mean;
   p.fmean0 = fmean0;
+  p.tscale0 = decorr_tscale(fmean0);
   p.cf.data = reinterpret_cast<const float2*>(f->cf.data);
   p.cf.offset = f->cf.offset;
   p.cf.support = f->cf.support;
@@ -688,6 +690,7 @@ void fill_params(KParams& p, const ddf_grid_desc* g, const ddf_vis* v, const ddf
   p.dp.uvw_dt = f->decorr.uvw_dt;
   p.dp.dt = f->decorr.dt;
   p.dp.dnu = f->decorr.dnu;
+  p.dp.fscale = decorr_fscale(f->decorr.dnu);
   p.dp.l0 = f->decorr.l0;
   p.dp.m0 = f->decorr.m0;
   p.sumwt = f->sumwt;
@@ -1393,7 +1396,9 @@ void fill_batch(BatchParams& p, const ddf_grid_desc* g, const ddf_facet* f0, con
   p.dp.uvw_dt = f0->decorr.uvw_dt;
   p.dp.dt = f0->decorr.dt;
   p.dp.dnu = f0->decorr.dnu;
+  p.dp.fscale = decorr_fscale(f0->decorr.dnu);
   p.fmean0 = fmean0;
+  p.tscale0 = decorr_tscale(fmean0);
   p.wl_mean = wl_mean;
 }
 
