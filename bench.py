@@ -741,6 +741,71 @@ def run_flags(a):
     print(json.dumps(res))
 
 
+def run_beams(a):
+    """Beam Jones producers on the device: the GMRT band-5 analytic E-Jones of 81 facet directions
+    (9 x 9 over 1.8 deg) for 64 antennas, 64 channels 1.05-1.45 GHz into 64 Jones channels, beam
+    intervals of 1 min over 600 x 8 s (80 intervals), CenterNorm on (ddf_beam_jones), then merged with
+    a DD-solution table of 10-min intervals and 8 channels (ddf_jones_merge), tables in HBM.
+    CPU baseline: the numpy restatement (oracle/beams.py) on the first 4 beam intervals."""
+    import torch
+    from ddfacet_amd import beams
+    from oracle import beams as ob
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    na, nchan = 64, 64
+    freqs = np.linspace(1.05e9, 1.45e9, nchan)
+    width = freqs[1] - freqs[0]
+    ra_c, dec_c = 1.2, -0.5
+    off = np.deg2rad(np.linspace(-0.9, 0.9, 9))
+    ra = (ra_c + off[:, None] / np.cos(dec_c) + 0 * off[None, :]).ravel()
+    dec = (dec_c + 0 * off[:, None] + off[None, :]).ravel()
+    times = np.arange(0.0, 600 * 8.0, 8.0)
+    tb = beams.beam_sample_times(times, 1)
+    t0d = np.arange(0.0, 600 * 8.0, 600.0)
+    dd = {"Jones": None, "t0": t0d, "t1": t0d + 600.0, "tm": t0d + 300.0,
+          "FreqDomains": beams.freq_domains(freqs, width, 8)}
+    gen = torch.Generator(device=dev).manual_seed(81)
+    dd["Jones"] = torch.randn((t0d.size, ra.size, na, 8, 2, 2), dtype=torch.complex64, device=dev, generator=gen)
+    for _ in range(a.warmup):
+        beam = beams.estimate_beam(tb, ra, dec, ra_c, dec_c, freqs, width, na, kind="GMRT", nband=nchan)
+        merged = beams.merge_jones(dd, beam)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        beam = beams.estimate_beam(tb, ra, dec, ra_c, dec_c, freqs, width, na, kind="GMRT", nband=nchan)
+        merged = beams.merge_jones(dd, beam)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    nb = beam["Jones"].numel() // 4
+    nm = merged["Jones"].numel() // 4
+    cpu = None
+    if not a.no_cpu:
+        tcpu = time.perf_counter()
+        ob.estimate_beam(tb[:5], ra, dec, ra_c, dec_c, freqs, width, nchan, na, "GMRT", center_norm=True)
+        dt = time.perf_counter() - tcpu
+        cpu = {"value": (4 * ra.size * na * nchan) / dt, "unit": "Jones matrices/s", "cores": 1, "kind": "port",
+               "sample": "numpy restatement (oracle/beams.py) of EstimateBeam on the first 4 beam intervals "
+                         "(%d matrices), %.2f s; the domain merge not included" % (4 * ra.size * na * nchan, dt)}
+    # algorithmic bytes: the beam table written once, both merge inputs read and the merged table
+    # written (complex64 2 x 2 matrices, 32 bytes)
+    nbytes = 32 * (nb + nb + dd["Jones"].numel() // 4 + nm)
+    res = {
+        "metric": "Jones matrices/s, beam E-Jones evaluation + merge with DD solutions",
+        "value": (nb + nm) / (ms * 1e-3), "unit": "Jones matrices/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 beam model, c64 tables", "data": "synthetic",
+        "config": {"workload": "GMRT band-5 beam of 81 directions x 64 antennas x 64 channels, 80 one-minute "
+                               "intervals, CenterNorm, merged with 10-min x 8-channel DD solutions",
+                   "beam_matrices": nb, "merged_matrices": nm},
+        "roofline": {"bound": "hbm", "kernel": "beam_write_k + jones_merge_k", "achieved": nbytes / (ms * 1e-3) / 1e9,
+                     "peak": 8000.0, "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / 8e12, "traffic": None,
+                     "algorithmic_bytes_per_launch": nbytes,
+                     "note": "end-to-end host calls (domain logic, model upload, synchronisation) included"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res))
+
+
 def run_weights(a):
     """Briggs imaging weights (robust 0) of the 81-facet workload's chunk on the device: uv-grid
     index, scatter-add, robust transform, renormalisation, inputs resident in HBM.  CPU baseline:
@@ -811,7 +876,7 @@ def main():
     ap.add_argument("--facets", type=int, default=0, help="use only the first N facets (debug)")
     ap.add_argument("--workload", default="meerkat81",
                     choices=["meerkat81", "single512", "uneven121", "autograd25", "weights", "flags", "bda",
-                             "pipeline"],
+                             "pipeline", "beams"],
                     help="meerkat81: the 81-facet MeerKAT-like cycle (the headline line); single512: the "
                          "plain single-facet workload, CPU and GPU plain / batched; uneven121: "
                          "121 facets x 4 bands with baseline-dependent averaging, plain per-facet "
@@ -820,7 +885,8 @@ def main():
                          "degridder on 25 facets x 4 bands; weights: Briggs imaging weights of the "
                          "81-facet workload's chunk; flags: flag preparation of that chunk; bda: the "
                          "chunk's BDA block map built on the device; pipeline: chunks uploaded from host memory "
-                         "on a copy stream while the previous chunk's cycle runs")
+                         "on a copy stream while the previous chunk's cycle runs; beams: analytic beam "
+                         "E-Jones tables and their merge with DD solutions")
     ap.add_argument("--tile", type=int, default=0, help="uv tile edge of the batched plan (0: the plan default)")
     ap.add_argument("--blocks-per-item", type=int, default=0,
                     help="BDA blocks per pass-A work item of the batched plan (0: the plan default)")
@@ -866,6 +932,8 @@ def main():
         return run_bda(a)
     if a.workload == "pipeline":
         return run_pipeline(a)
+    if a.workload == "beams":
+        return run_beams(a)
     from ddfacet_amd import configs
     t0 = time.time()
     # geometry, maps, CFs and Jones; the CPU baseline runs before the GPU is touched, so with it

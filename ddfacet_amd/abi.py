@@ -185,7 +185,23 @@ class ddf_bda_spec(C.Structure):
                 ("k_phase", C.c_double), ("lmax", C.c_double), ("nmax", C.c_double), ("k_span", C.c_double),
                 ("dphi", C.c_double)]
 
+class ddf_beam_desc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("nchan", C.c_int32), ("ncoef", C.c_int32), ("nchan_jones", C.c_int32),
+                ("freqs", C.c_void_p), ("coef0", C.c_void_p), ("coef1", C.c_void_p), ("band_f0", C.c_void_p),
+                ("x_null", C.c_void_p), ("y_null", C.c_void_p), ("chan_to_jones", C.c_void_p)]
+
+
+class ddf_jones_merge_desc(C.Structure):
+    _fields_ = [("j0", C.c_void_p), ("j1", C.c_void_p), ("out", C.c_void_p), ("it0", C.c_void_p),
+                ("it1", C.c_void_p), ("ich0", C.c_void_p), ("ich1", C.c_void_p), ("nt0", C.c_int32),
+                ("nch0", C.c_int32), ("nt1", C.c_int32), ("nch1", C.c_int32), ("nd", C.c_int32), ("na", C.c_int32),
+                ("nt", C.c_int32), ("nch", C.c_int32)]
+
+
 EXTRA_SIGNATURES = {
+    "ddf_beam_jones": [C.POINTER(ddf_beam_desc), C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_double,
+                       C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p],
+    "ddf_jones_merge": [C.POINTER(ddf_jones_merge_desc), C.c_void_p],
     "ddf_weights_index": [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_int64,
                           C.c_int64, C.c_int, C.c_int64, C.c_void_p, C.c_void_p],
     "ddf_prepare_flags": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_void_p],

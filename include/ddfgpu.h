@@ -377,6 +377,53 @@ typedef struct ddf_image_facet {
   const double* sum_jones;  /* float64 [2][nband], or NULL without Jones                         */
 } ddf_image_facet;
 
+/* ---- Beam Jones producers ------------------------------------------------------------------------
+ * ddf_beam_jones: the analytic primary-beam E-Jones of ClassJones.EstimateBeam
+ * (DDFacet/Data/ClassJones.py:1014-1065) for the GMRT and ATCA models
+ * (ClassGMRTBeam.GiveRawBeam / GiveInstrumentBeam, DDFacet/Data/ClassGMRTBeam.py:140-173;
+ * ClassATCABeam.GiveRawBeam / GiveInstrumentBeam, DDFacet/Data/ClassATCABeam.py:191-245): the beam
+ * of each direction (ra, dec) from the phase centre (ra_c, dec_c), averaged over the channels of
+ * each Jones channel, optionally normalised by the inverse of the beam at the phase centre
+ * (CenterNorm), written as complex64 [ntime][ndir][na][nchan_jones][2][2] -- the layout of the
+ * ddf_jones tables, so the output feeds the gridders directly.  The analytic models depend on
+ * neither time nor antenna; ntime and na set the table's shape.  All pointers are device memory.
+ * Per-channel model terms (ascending polynomial coefficients in float32-rounded values, null
+ * thresholds) come from the host (ddfacet_amd/beams.py restates calcCoefs).                     */
+enum { DDF_BEAM_GMRT = 1, DDF_BEAM_ATCA = 2 };
+typedef struct ddf_beam_desc {
+  int32_t kind;            /* DDF_BEAM_GMRT or DDF_BEAM_ATCA */
+  int32_t nchan;           /* visibility channels */
+  int32_t ncoef;           /* coefficients per polynomial, <= 16 */
+  int32_t nchan_jones;
+  const double* freqs;     /* [nchan] Hz */
+  const double* coef0;     /* [nchan][ncoef]: GMRT the channel's band; ATCA the band at or below */
+  const double* coef1;     /* [nchan][ncoef]: ATCA the band at or above (GMRT: unused) */
+  const double* band_f0;   /* [nchan][2]: ATCA the two bands' frequencies (GMRT: unused) */
+  const float* x_null;     /* [nchan]: GMRT arcmin; ATCA arcmin GHz */
+  const float* y_null;     /* [nchan]: GMRT the clipped value (ATCA: computed here) */
+  const int32_t* chan_to_jones; /* [nchan] */
+} ddf_beam_desc;
+int ddf_beam_jones(const ddf_beam_desc* b, const double* ra, const double* dec, int32_t ndir, double ra_c,
+                   double dec_c, int32_t ntime, int32_t na, int32_t center_norm, void* jones, void* stream);
+
+/* ddf_jones_merge: the cell products of ClassJonesDomains.MergeJones
+ * (DDFacet/Other/ClassJonesDomains.py:142-212, behind ClassJones.MergeJones, ClassJones.py:1067-1072):
+ * out[t][d][a][c] = J0[it0[t]][d][a][ich0[c]] . J1[it1[t]][d][a][ich1[c]] (2 x 2 complex64
+ * products, ModLinAlg.BatchDot), the identity where ich0[c] or ich1[c] is -1.  The merged time and
+ * frequency domains and the index maps (it*: nearest centre time, ich*: containing channel) are
+ * small host arrays (ddfacet_amd/beams.py); all pointers are device memory.                      */
+typedef struct ddf_jones_merge_desc {
+  const void* j0;          /* complex64 [nt0][nd][na][nch0][2][2] */
+  const void* j1;          /* complex64 [nt1][nd][na][nch1][2][2] */
+  void* out;               /* complex64 [nt][nd][na][nch][2][2] */
+  const int32_t* it0;      /* [nt] */
+  const int32_t* it1;      /* [nt] */
+  const int32_t* ich0;     /* [nch] */
+  const int32_t* ich1;     /* [nch] */
+  int32_t nt0, nch0, nt1, nch1, nd, na, nt, nch;
+} ddf_jones_merge_desc;
+int ddf_jones_merge(const ddf_jones_merge_desc* m, void* stream);
+
 /* Centred 2-D FFT of nplanes n x n complex64 planes in place, ModFFTW.FFTW_2Donly
  * (DDFacet/ToolsDir/ModFFTW.py:557-600): forward fftshift(fft2(ifftshift(A))) / n^2, inverse
  * fftshift(ifft2(ifftshift(A))) * n^2, both times `scale`.  GridToIm

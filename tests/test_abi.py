@@ -37,7 +37,8 @@ def test_oracle_library_loads():
 
 
 STRUCTS = ["ddf_cf", "ddf_jones", "ddf_decorr", "ddf_vis", "ddf_facet", "ddf_grid_desc", "ddf_plan_opts",
-           "ddf_image_facet", "ddf_cf_spec"]
+           "ddf_image_facet", "ddf_cf_spec", "ddf_flag_select", "ddf_bda_spec", "ddf_beam_desc",
+           "ddf_jones_merge_desc"]
 
 
 def test_ctypes_layout_matches_c_header():
