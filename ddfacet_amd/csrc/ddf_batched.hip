@@ -22,6 +22,7 @@
 //           Jones, phasor, decorrelation) in registers and writes its visibility once.
 // Work queues are device counters reset by a small kernel (no memset nodes, capture-safe).
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <cstring>
 #include <type_traits>
@@ -179,6 +180,32 @@ __global__ __launch_bounds__(256) void plan_fill(BatchParams p, int64_t nblocks,
     uint32_t slot = atomicAdd(&fill[bin], 1u);
     recs[bin_start[bin] + slot] = uint32_t(t);
   }
+}
+
+// Order of the records inside a work item: by CF plane, then oversampled sub-kernel, then record,
+// so that the lanes of a wavefront (consecutive records) read the same sub-kernels (one cache line
+// for a cell instead of one per lane in degrid pass A, L1 hits in grid pass B), and the order does
+// not depend on the atomics of plan_fill.  Key = plane << 43 | sub << 32 | record.
+__global__ __launch_bounds__(256) void plan_sort_keys(BatchParams p, const uint64_t* place, const uint32_t* recs,
+                                                      int64_t n, uint64_t* keys) {
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t t = recs[k];
+    const Unpacked u = unpack_place(place[t]);
+    const uint64_t sub = uint64_t(u.io * p.overs + u.jo);
+    keys[k] = (uint64_t(u.plane) << 43) | (sub << 32) | uint64_t(t);
+  }
+}
+
+__global__ __launch_bounds__(256) void plan_item_bounds(const BinItem* items, uint32_t n, uint32_t* begin, uint32_t* end) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    begin[i] = items[i].begin;
+    end[i] = items[i].end;
+  }
+}
+
+__global__ __launch_bounds__(256) void plan_keys_to_recs(const uint64_t* keys, int64_t n, uint32_t* recs) {
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x)
+    recs[k] = uint32_t(keys[k]);
 }
 
 // Stamp records of the binned records (record k of the bin list): see Stamp.
@@ -1083,9 +1110,12 @@ __global__ __launch_bounds__(256) void grid_slot_reduce(BatchParams p, const flo
 // and mixed precisions, float in the FP32 precision).  No cross-lane work: the records of a bin
 // share their CF plane, so the lanes of a wavefront mostly share S and run in step.
 // ONE: a single Stokes plane (nstokes == 1), resolved at compile time.
-template <typename ACC, bool ONE>
+// PM: the precision (0 mixed: FP32 products summed in FP64; 1 FP64: FP64 products and sums, as
+// the reference degridder; 2 FP32 throughout).  ONE: a single Stokes plane.
+template <int PM, bool ONE>
 __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64_t* bstamp, float2* M,
                                                      const BinItem* items, const uint32_t* recs, uint32_t* queue) {
+  using ACC = typename std::conditional<PM == 2, float, double>::type;
   extern __shared__ __attribute__((aligned(16))) float tile_f[];
   float2* tile = reinterpret_cast<float2*>(tile_f);  // [nstokes][W * W]
   __shared__ uint32_t s_item;
@@ -1095,7 +1125,7 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
   while (true) {
     if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
     __syncthreads();
-    const uint32_t it = s_item;
+    const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
     if (it >= p.nitems_b) break;
     const BinItem bi = items[it];
     const FacetDev fd = gload(p.facets + bi.facet);
@@ -1112,42 +1142,45 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
       tile[i] = m;
     }
     __syncthreads();
+    const int O2 = p.overs * p.overs;
     for (uint32_t k = bi.begin + threadIdx.x; k < bi.end; k += blockDim.x) {
       const uint64_t rec = bstamp[k];
       const uint32_t t = recs[k];
       const int S = int((rec >> 16) & 0x1F);
-      const int S2 = S * S;
       // cell c of sub-kernel `sub` at plane offset + c * overs^2 + sub of the transposed planes:
       // the lanes of a wavefront (records of one plane) read neighbouring words
       const int sub = int((rec >> 21) & 0x7FF);
-      const int O2 = p.overs * p.overs;
-      const DDF_GLOBAL float2* cp = cft + (int64_t(uint32_t(rec >> 32)) - int64_t(sub) * S2 + sub);
+      const DDF_GLOBAL float2* cp = cft + (int64_t(uint32_t(rec >> 32)) - int64_t(sub) * S * S + sub);
+      const int toff0 = int(rec & 0xFFFF);  // tile offset of the stamp's first cell
       ACC sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
-      int col = 0, toff = int(rec & 0xFFFF);  // column and tile offset of the next cell
-      for (int c = 0; c < S2; c += 4) {
-        float2 w[4];
-        int off[4];
+      for (int r = 0; r < S; ++r) {
+        const DDF_GLOBAL float2* crow = cp + int64_t(r * S) * O2;
+        const float2* trow = tile + toff0 + r * W;
+        // four cells at a time, their CF loads issued together
+        for (int c = 0; c < S; c += 4) {
+          float2 w[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const bool in = c + j < S2;
-          w[j] = in ? ld2(cp + int64_t(c + j) * O2) : make_float2(0.f, 0.f);
-          off[j] = in ? toff : -1;
-          if (++col == S) {
-            col = 0;
-            toff += W - S + 1;
-          } else {
-            ++toff;
-          }
-        }
+          for (int j = 0; j < 4; ++j) w[j] = c + j < S ? ld2(crow + int64_t(c + j) * O2) : make_float2(0.f, 0.f);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          if (off[j] < 0) continue;  // past the stamp (last group of 4)
+          for (int j = 0; j < 4; ++j) {
+            if (c + j >= S) break;
 #pragma unroll
-          for (int ip = 0; ip < 4; ++ip) {
-            if (ip < ns) {
-              const float2 m = tile[ip * WW + off[j]];
-              sr[ip] += ACC(m.x * w[j].x - m.y * w[j].y);
-              si[ip] += ACC(m.x * w[j].y + m.y * w[j].x);
+            for (int ip = 0; ip < 4; ++ip) {
+              if (ip < ns) {
+                const float2 m = trow[ip * WW + c + j];
+                if (PM == 0) {
+                  sr[ip] += ACC(m.x * w[j].x - m.y * w[j].y);
+                  si[ip] += ACC(m.x * w[j].y + m.y * w[j].x);
+                } else if (PM == 1) {
+                  sr[ip] += double(m.x) * double(w[j].x) - double(m.y) * double(w[j].y);
+                  si[ip] += double(m.x) * double(w[j].y) + double(m.y) * double(w[j].x);
+                } else {
+                  sr[ip] = fmaf(m.x, w[j].x, sr[ip]);
+                  sr[ip] = fmaf(-m.y, w[j].y, sr[ip]);
+                  si[ip] = fmaf(m.x, w[j].y, si[ip]);
+                  si[ip] = fmaf(m.y, w[j].x, si[ip]);
+                }
+              }
             }
           }
         }
@@ -1574,6 +1607,45 @@ int batched_fill(const BatchParams& p, int64_t nblocks, const uint64_t* place, c
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
+// Records of every work item sorted by (CF plane, sub-kernel, record): one segmented radix sort
+// of 51-bit keys over the items' ranges (a plane needs 8 bits, a sub-kernel 11, a record 32).
+int batched_sort_items(const BatchParams& p, const uint64_t* place, uint32_t* recs, int64_t n, const BinItem* items,
+                       uint32_t nitems, hipStream_t st) {
+  if (n <= 0 || nitems == 0) return DDF_OK;
+  if (n >= (int64_t(1) << 31)) return DDF_EINVAL;
+  uint64_t *k0 = nullptr, *k1 = nullptr;
+  uint32_t *b0 = nullptr, *b1 = nullptr;
+  void* tmp = nullptr;
+  size_t tb = 0;
+  int rc = DDF_OK;
+  auto grid = [](int64_t m) { int64_t g = (m + 255) / 256; return unsigned(g > 65536 ? 65536 : (g < 1 ? 1 : g)); };
+  if (hipMallocAsync(reinterpret_cast<void**>(&k0), size_t(n) * 8, st) != hipSuccess ||
+      hipMallocAsync(reinterpret_cast<void**>(&k1), size_t(n) * 8, st) != hipSuccess ||
+      hipMallocAsync(reinterpret_cast<void**>(&b0), size_t(nitems) * 4, st) != hipSuccess ||
+      hipMallocAsync(reinterpret_cast<void**>(&b1), size_t(nitems) * 4, st) != hipSuccess) {
+    rc = DDF_ENOMEM;
+  } else {
+    hipLaunchKernelGGL(plan_sort_keys, dim3(grid(n)), dim3(256), 0, st, p, place, recs, n, k0);
+    hipLaunchKernelGGL(plan_item_bounds, dim3(grid(nitems)), dim3(256), 0, st, items, nitems, b0, b1);
+    if (hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, k0, k1, int(n), int(nitems), b0, b1, 0, 51, st) !=
+            hipSuccess ||
+        hipMallocAsync(&tmp, tb ? tb : 1, st) != hipSuccess ||
+        hipcub::DeviceSegmentedRadixSort::SortKeys(tmp, tb, k0, k1, int(n), int(nitems), b0, b1, 0, 51, st) !=
+            hipSuccess) {
+      rc = DDF_EHIP;
+    } else {
+      hipLaunchKernelGGL(plan_keys_to_recs, dim3(grid(n)), dim3(256), 0, st, k1, n, recs);
+      if (hipGetLastError() != hipSuccess) rc = DDF_EHIP;
+    }
+  }
+  if (tmp) hipFreeAsync(tmp, st);
+  if (k0) hipFreeAsync(k0, st);
+  if (k1) hipFreeAsync(k1, st);
+  if (b0) hipFreeAsync(b0, st);
+  if (b1) hipFreeAsync(b1, st);
+  return rc;
+}
+
 int batched_vis_counts(const VisP& v, int32_t* cnt, hipStream_t st) {
   int64_t nb = (v.nblocks + 255) / 256;
   if (nb > 65536) nb = 65536;
@@ -1647,8 +1719,6 @@ static unsigned persistent_wgs(K kernel, size_t lds, unsigned nwg, int block = 2
   return unsigned(ncu * per);
 }
 
-// Persistent grid sizes of the tile passes (grid B, degrid A, adjoint of degrid A), computed once
-// per plan so that the launches make no device queries (they may run under stream capture).
 // grid_pass_b instance for a tile pass: CONJ, one polarisation or several, stamps beyond 256 cells.
 typedef void (*GridPassB)(BatchParams, const uint64_t*, const float2*, const BinItem*, const uint32_t*, uint32_t*,
                           float2*);
@@ -1659,6 +1729,14 @@ static GridPassB grid_pass_b_for(int npg, int halo) {
   return big ? grid_pass_b<CONJ, false, true> : grid_pass_b<CONJ, false, false>;
 }
 
+// degrid_pass_a instance for a precision (0 mixed, 1 FP64, 2 FP32) and one or several Stokes planes.
+typedef void (*DegridPassA)(BatchParams, const uint64_t*, float2*, const BinItem*, const uint32_t*, uint32_t*);
+static DegridPassA degrid_pass_a_for(int prec, bool one) {
+  if (prec == 1) return one ? degrid_pass_a<1, true> : degrid_pass_a<1, false>;
+  if (prec == 2) return one ? degrid_pass_a<2, true> : degrid_pass_a<2, false>;
+  return one ? degrid_pass_a<0, true> : degrid_pass_a<0, false>;
+}
+
 // Persistent grid sizes of the tile passes (grid B, degrid A, adjoint of degrid A), computed once
 // per plan so that the launches make no device queries (they may run under stream capture).
 int batched_tile_wgs(const BatchParams& pg, const BatchParams& pd, unsigned out[3]) {
@@ -1667,12 +1745,7 @@ int batched_tile_wgs(const BatchParams& pg, const BatchParams& pd, unsigned out[
   const int npd = pd.nstokes < pd.npol ? pd.nstokes : pd.npol;
   out[0] = persistent_wgs(grid_pass_b_for<false>(npg, pg.halo), size_t(npg) * Wg * Wg * sizeof(float2), 0, 64);
   const size_t lds_d = size_t(pd.nstokes) * Wd * Wd * 2 * sizeof(float);
-  if (pd.nstokes == 1)
-    out[1] = pd.prec != 2 ? persistent_wgs(degrid_pass_a<double, true>, lds_d, 0)
-                          : persistent_wgs(degrid_pass_a<float, true>, lds_d, 0);
-  else
-    out[1] = pd.prec != 2 ? persistent_wgs(degrid_pass_a<double, false>, lds_d, 0)
-                          : persistent_wgs(degrid_pass_a<float, false>, lds_d, 0);
+  out[1] = persistent_wgs(degrid_pass_a_for(pd.prec, pd.nstokes == 1), lds_d, 0);
   out[2] = persistent_wgs(grid_pass_b_for<true>(npd, pd.halo), size_t(pd.nstokes) * Wd * Wd * sizeof(float2), 0, 64);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
@@ -1710,24 +1783,13 @@ int batched_degrid_a_adjoint(const BatchParams& p, const uint64_t* place, const 
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
-template <typename ACC, bool ONE>
-static void launch_degrid_a(const BatchParams& p, const uint64_t* place, float2* M, const BinItem* items,
-                            const uint32_t* recs, uint32_t* queue, unsigned nwg, size_t lds, hipStream_t st) {
-  nwg = persistent_wgs(degrid_pass_a<ACC, ONE>, lds, nwg);
-  hipLaunchKernelGGL((degrid_pass_a<ACC, ONE>), dim3(nwg), dim3(256), lds, st, p, place, M, items, recs, queue);
-}
-
 int batched_degrid_a(const BatchParams& p, const uint64_t* place, float2* M, const BinItem* items,
                      const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st) {
   const int W = p.tile + 2 * p.halo;
-  size_t lds = size_t(p.nstokes) * W * W * 2 * sizeof(float);
-  if (p.prec != 2) {  // FP64 stamp sums (as the reference) except in the FP32 precision
-    if (p.nstokes == 1) launch_degrid_a<double, true>(p, place, M, items, recs, queue, nwg, lds, st);
-    else launch_degrid_a<double, false>(p, place, M, items, recs, queue, nwg, lds, st);
-  } else {
-    if (p.nstokes == 1) launch_degrid_a<float, true>(p, place, M, items, recs, queue, nwg, lds, st);
-    else launch_degrid_a<float, false>(p, place, M, items, recs, queue, nwg, lds, st);
-  }
+  const size_t lds = size_t(p.nstokes) * W * W * 2 * sizeof(float);
+  const DegridPassA k = degrid_pass_a_for(p.prec, p.nstokes == 1);
+  nwg = persistent_wgs(k, lds, nwg);
+  hipLaunchKernelGGL(k, dim3(nwg), dim3(256), lds, st, p, place, M, items, recs, queue);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 

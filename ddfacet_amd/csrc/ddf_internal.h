@@ -217,6 +217,8 @@ int batched_block_desc(const VisP& v, uint4* desc, hipStream_t st);
 int batched_count(const BatchParams& p, int64_t nblocks, const uint64_t* place, uint32_t* counts, hipStream_t st);
 int batched_fill(const BatchParams& p, int64_t nblocks, const uint64_t* place, const int64_t* bin_start,
                  uint32_t* fill, uint32_t* recs, hipStream_t st);
+int batched_sort_items(const BatchParams& p, const uint64_t* place, uint32_t* recs, int64_t n, const BinItem* items,
+                       uint32_t nitems, hipStream_t st);
 int batched_vis_counts(const VisP& v, int32_t* cnt, hipStream_t st);
 int batched_vlist(const VisP& v, const int64_t* vis_start, uint64_t* vrc, uint32_t* vblk, hipStream_t st);
 int batched_reset(uint32_t* counters, int n, hipStream_t st);

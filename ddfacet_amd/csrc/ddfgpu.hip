@@ -1492,6 +1492,8 @@ int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place
     *tile_count_out = tc;
   }
   if (batched_fill(p, v.nblocks, place, bstart, fill, recs, st)) return fail(DDF_EHIP, "plan_fill launch failed");
+  if (batched_sort_items(p, place, recs, tot, d_items, uint32_t(items.size()), st))
+    return fail(DDF_EHIP, "sorting the records of the work items failed");
   HIP_TRY(hipStreamSynchronize(st));
   *recs_out = recs;
   *items_out = d_items;
