@@ -642,9 +642,12 @@ def run_autograd(a, gd_steps=20):
     # adjoint check on random vectors
     xr = torch.randn(shape, generator=gen, device=dev)
     yr = torch.randn(data.shape, generator=gen, device=dev, dtype=torch.complex64)
-    lhs = torch.sum(op.forward(xr).to(torch.complex128).conj() * yr.to(torch.complex128)).real
+    axr = op.forward(xr).to(torch.complex128)
+    lhs = torch.sum(axr.conj() * yr.to(torch.complex128)).real
     rhs = torch.sum(xr.double() * op.adjoint(yr).double())
-    dot_rel = float(abs(lhs - rhs) / max(abs(float(lhs)), 1e-30))
+    # relative to the Cauchy-Schwarz scale |A x| |y| (the inner product of random vectors is far below it)
+    cs = torch.linalg.vector_norm(axr) * torch.linalg.vector_norm(yr.to(torch.complex128))
+    dot_rel = float(abs(lhs - rhs) / max(float(cs), 1e-30))
     # step 1 / L, L = 2 lambda_max(A^H W A) by power iteration
     y = xr / xr.norm()
     lam = 0.0

@@ -182,17 +182,31 @@ __global__ __launch_bounds__(256) void plan_fill(BatchParams p, int64_t nblocks,
   }
 }
 
-// Order of the records inside a work item: by CF plane, then oversampled sub-kernel, then record,
-// so that the lanes of a wavefront (consecutive records) read the same sub-kernels (one cache line
-// for a cell instead of one per lane in degrid pass A, L1 hits in grid pass B), and the order does
-// not depend on the atomics of plan_fill.  Key = plane << 43 | sub << 32 | record.
+// Order of the records inside a work item, independent of the atomics of plan_fill.  Degridding
+// (STAMP false): by CF plane, then oversampled sub-kernel, then record, so that the lanes of a
+// wavefront (consecutive records) read the same sub-kernels (one cache line for a cell instead of
+// one per lane); key = plane << 43 | sub << 32 | record.  Gridding (STAMP true): by support, then
+// stamp position in the tile image, then record, so that the stamps of one position come in runs
+// that grid pass B sums in registers; key = S << 46 | base << 32 | record (base taken to 14 bits:
+// tile images that fit LDS have fewer cells, and grid pass B finds runs from the full stamp
+// record, so the key only decides the order).
+template <bool STAMP>
 __global__ __launch_bounds__(256) void plan_sort_keys(BatchParams p, const uint64_t* place, const uint32_t* recs,
                                                       int64_t n, uint64_t* keys) {
+  const int W = p.tile + 2 * p.halo;
   for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x) {
     const uint32_t t = recs[k];
     const Unpacked u = unpack_place(place[t]);
-    const uint64_t sub = uint64_t(u.io * p.overs + u.jo);
-    keys[k] = (uint64_t(u.plane) << 43) | (sub << 32) | uint64_t(t);
+    if (STAMP) {
+      const int S = plane_support(p, u.plane);
+      const int sup = (S - 1) / 2;
+      const int x0 = (u.locx / p.tile) * p.tile - p.halo, y0 = (u.locy / p.tile) * p.tile - p.halo;
+      const uint64_t base = uint64_t((u.locy - sup - y0) * W + (u.locx - sup - x0)) & 0x3FFF;
+      keys[k] = (uint64_t(S & 0x1F) << 46) | (base << 32) | uint64_t(t);
+    } else {
+      const uint64_t sub = uint64_t(u.io * p.overs + u.jo);
+      keys[k] = (uint64_t(u.plane) << 43) | (sub << 32) | uint64_t(t);
+    }
   }
 }
 
@@ -319,6 +333,21 @@ __global__ void plan_block_desc(const VisP v, uint4* desc) {
 
 __global__ void reset_counter(uint32_t* c, int n) {
   if (threadIdx.x < n) c[threadIdx.x] = 0;
+}
+
+// Next work item of a tile pass for a workgroup, taken by one thread whose state (x, left) starts
+// at (blockIdx.x % kItemRanges, kItemRanges).  Workgroups are dispatched to the 8 XCDs round robin,
+// so the workgroups of one XCD take the items of their own range x (one counter per range) in
+// order; a workgroup whose range is used up moves on to the next ranges.  Returns 0xFFFFFFFF once
+// it has found every range used up (at most kItemRanges failed atomics per workgroup).
+__device__ __forceinline__ uint32_t next_tile_item(uint32_t* queue, const uint32_t* range, int& x, int& left) {
+  while (left > 0) {
+    const uint32_t it = range[x] + atomicAdd(queue + x, 1u);
+    if (it < range[x + 1]) return it;
+    x = x + 1 == kItemRanges ? 0 : x + 1;
+    --left;
+  }
+  return 0xFFFFFFFFu;
 }
 
 // ---- cycle: grid pass A --------------------------------------------------------------------------
@@ -863,16 +892,15 @@ __device__ __forceinline__ int stamp_support(uint64_t srec) { return int((srec >
 __device__ __forceinline__ int stamp_base(uint64_t srec) { return int(srec & 0xFFFF); }
 
 // Per-lane cell layout of a stamp of support S for the first 256 cells (cell c = 64 r + lane):
-// off[r] its tile offset from the stamp's first cell (-1 past the stamp), idx8[r] the byte offset
-// of the CF value it reads (cell 0 past the stamp, so that those loads stay inside the sub-kernel).
-// It depends on S only, and the records of a work item are sorted by CF plane, so a pass recomputes
-// it only when S changes.  (c + 0.5) / S is at least 0.5 / S from an integer, far above the error
-// of (c + 0.5) * rcp(S) in FP32 for c < 1024, S < 32 (below 1e-3), so the row is exact.
+// off[r] its tile offset from the stamp's first cell (kOffOut past the stamp).  It depends on S
+// only, and the records of a work item are sorted by support, so a pass recomputes it only when S
+// changes.  (c + 0.5) / S is at least 0.5 / S from an integer, far above the error of
+// (c + 0.5) * rcp(S) in FP32 for c < 1024, S < 32 (below 1e-3), so the row is exact.
+constexpr uint32_t kOffOut = 1u << 30;
 template <int G>
 struct StampLayout {
   int S;
-  int off[G];
-  uint32_t idx8[G];
+  uint32_t off[G];
 };
 template <int G>
 __device__ __forceinline__ void stamp_layout(int S, int lane, int W, StampLayout<G>& L) {
@@ -883,135 +911,223 @@ __device__ __forceinline__ void stamp_layout(int S, int lane, int W, StampLayout
   for (int r = 0; r < G; ++r) {
     const int c = 64 * r + lane;
     const int sy = int((float(c) + 0.5f) * invS);
-    const bool in = c < S2;
-    L.off[r] = in ? sy * W + (c - sy * S) : -1;
-    L.idx8[r] = in ? uint32_t(c) * uint32_t(sizeof(float2)) : 0u;
+    L.off[r] = c < S2 ? uint32_t(sy * W + (c - sy * S)) : kOffOut;
   }
+}
+
+// x += (vr + i vi) (wx + i wy), in the tile's precision: FP32 FMAs, or in FP64 the exact products
+// of the FP32 operands summed in FP64.
+__device__ __forceinline__ void cmac(float2& x, float vr, float vi, float wx, float wy) {
+  x.x = fmaf(vr, wx, x.x);
+  x.x = fmaf(-vi, wy, x.x);
+  x.y = fmaf(vr, wy, x.y);
+  x.y = fmaf(vi, wx, x.y);
+}
+__device__ __forceinline__ void cmac(double2& x, float vr, float vi, float wx, float wy) {
+  x.x = fma(double(vr), double(wx), x.x);
+  x.x = fma(-double(vi), double(wy), x.x);
+  x.y = fma(double(vr), double(wy), x.y);
+  x.y = fma(double(vi), double(wx), x.y);
 }
 
 // ---- cycle: grid pass B (LDS-tiled scatter) ------------------------------------------------------
 // One single-wavefront workgroup per bin item, with a private LDS tile image (T + 2 h)^2 per
-// polarisation, (re, im) interleaved.  The wavefront loads 64 records (stamp record, value) at a
-// time, then adds their stamps one after the other with the record in scalar registers; lanes are
-// stamp cells, G groups of 64 cells per record, with G the largest ceil(S^2 / 64) of the 64
-// records (records are sorted by CF plane, so mostly that of every record).  Every record issues
-// exactly G CF loads, so the number of loads in flight is known at compile time and the wait before
-// a record's adds covers its own loads only: the CF values of the records two ahead are loaded
-// while the current one is added.  Addresses: the record's sub-kernel and tile position are
-// scalars, the lane's cell offsets come from the support's StampLayout (kept across records of one
-// support), so a cell costs an address add and the complex multiply-add.  The adds are plain LDS
-// read-modify-writes: the cells of one stamp are distinct lanes' and a wavefront's LDS operations
-// complete in issue order, so no atomics are needed.  The tile then goes to its slot (or, without
+// polarisation, (re, im) interleaved.  The records of an item are sorted by (support, stamp
+// position, record), so records with the same stamp come in runs (~30 on the 81-facet workload).
+// The wavefront loads 64 records (stamp record, value) at a time, then takes them one after the
+// other with the record in scalar registers; lanes are stamp cells, G groups of 64 cells per
+// record, with G the largest ceil(S^2 / 64) of the 64 records.  A record's stamp (value times its
+// CF sub-kernel) is added to the run's sums in registers; when the stamp position changes, the
+// sums go into the LDS tile with one read-modify-write per cell.  Every record issues exactly G CF
+// loads, so the number of loads in flight is known at compile time and the wait before a record's
+// adds covers its own loads only: the CF values of the next D - 1 records (7, 3 or 2 for G = 1, 2,
+// 3-4) are in flight while the current one is added.  Addresses: the record's sub-kernel and tile position are scalars, the
+// lane's cell offsets come from the support's StampLayout (kept across records of one support).
+// The read-modify-writes need no atomics: the cells of one stamp are distinct lanes' and a
+// wavefront's LDS operations complete in issue order.  The tile then goes to its slot (or, without
 // slots, into the facet grid with global float atomics on the non-zero cells).
-template <bool CONJ, bool ONE, bool BIG, int G>
-__device__ __forceinline__ void grid_b_batch(float2* tile, int W, int WW, int npg, int lane, const DDF_GLOBAL char* cf,
-                                             uint64_t my_rec, const float2 (&my_v)[4], int cnt) {
-  StampLayout<G> Lload, Ladd;
-  Lload.S = Ladd.S = -1;
-  uint64_t rA = 0, rB = 0, rC = 0;
-  float2 vA[G], vB[G], vC[G];
-  // records past the batch load the last one again (their values are not used)
-  auto load = [&](int j, uint64_t& rec, float2 (&v)[G]) {
-    rec = lane_u64(my_rec, j < cnt ? j : cnt - 1);
-    const int S = stamp_support(rec);
-    if (S != Lload.S) stamp_layout<G>(S, lane, W, Lload);
-    const DDF_GLOBAL char* cfp = cf + (size_t(uint32_t(rec >> 32)) << 3);
+template <bool CONJ, bool ONE, bool BIG, int G, bool FULL, typename T2>
+__device__ __forceinline__ void grid_b_batch(T2* tile, int W, int WW, int npg, int lane, const DDF_GLOBAL char* cf,
+                                             uint32_t my_o8, uint32_t my_lim8, uint32_t my_key,
+                                             const float2 (&my_v)[4], int cnt) {
+  constexpr int NP = ONE ? 1 : 4;
+  // records in flight: D - 1 ahead of the one being added (D divides 64)
+  constexpr int D = G == 1 ? 8 : 4;
+  StampLayout<G> Lrun;
+  Lrun.S = -1;
+  uint32_t c8[G];  // byte offset of the lane's cell in a sub-kernel, per group
 #pragma unroll
-    for (int r = 0; r < G; ++r) v[r] = ld2((const DDF_GLOBAL float2*)(cfp + Lload.idx8[r]));
+  for (int r = 0; r < G; ++r) c8[r] = uint32_t(64 * r + lane) * uint32_t(sizeof(float2));
+  uint32_t oq[D];
+  float2 vq[D][G];
+  // the open run: its stamp (support << 16 | base, -1 for none) and its register sums
+  int run = -1;
+  T2 acc[G][NP];
+  // Record k's CF values: cell c of its sub-kernel (byte offset o8 in the facet's CF buffer), a
+  // lane past the stamp reading the last cell (byte offset lim8) again.  Records past a partial
+  // batch load the last one again (their values are not used).
+  auto load = [&](int k, uint32_t& o, float2 (&v)[G]) {
+    const int kk = FULL ? k : min(k, cnt - 1);
+    o = lane_u32(my_o8, kk);
+    const uint32_t lim8 = lane_u32(my_lim8, kk);
+#pragma unroll
+    for (int r = 0; r < G; ++r) v[r] = ld2((const DDF_GLOBAL float2*)(cf + (min(c8[r], lim8) + o)));
   };
-  auto add = [&](int j, uint64_t rec, const float2 (&v)[G]) {
-    if (j >= cnt) return;
-    float vr[4], vi[4];
+  // the open run's sums into the tile: one read-modify-write per cell
+  auto flush = [&]() {
+    if (run < 0) return;
+    const int base = run & 0xFFFF;
+    if (ONE) {
+      // a lane past the stamp adds to its own spare cell after the tile (WW + lane, never read
+      // back) instead of being masked off; the G reads are issued before the adds
+      uint32_t a[G];
+      T2 x[G];
 #pragma unroll
-    for (int ip = 0; ip < 4; ++ip) {
+      for (int r = 0; r < G; ++r) {
+        a[r] = min(uint32_t(base) + Lrun.off[r], uint32_t(WW + lane));
+        x[r] = tile[a[r]];
+      }
+#pragma unroll
+      for (int r = 0; r < G; ++r) {
+        x[r].x += acc[r][0].x;
+        x[r].y += acc[r][0].y;
+        tile[a[r]] = x[r];
+      }
+    } else {
+      const int S2 = Lrun.S * Lrun.S;
+      T2* tb = tile + base;
+#pragma unroll
+      for (int r = 0; r < G; ++r) {
+        if (64 * r >= S2) break;
+        if (Lrun.off[r] != kOffOut) {
+          T2* t = tb + Lrun.off[r];
+#pragma unroll
+          for (int ip = 0; ip < NP; ++ip) {
+            if (ip < npg) {
+              T2 x = t[ip * WW];
+              x.x += acc[r][ip].x;
+              x.y += acc[r][ip].y;
+              t[ip * WW] = x;
+            }
+          }
+        }
+      }
+    }
+  };
+  auto add = [&](int j, uint32_t o8, const float2 (&v)[G]) {
+    if (!FULL && j >= cnt) return;
+    float vr[NP], vi[NP];
+#pragma unroll
+    for (int ip = 0; ip < NP; ++ip) {
       if (ip < npg) {
         vr[ip] = lane_f32(my_v[ip].x, j);
         vi[ip] = lane_f32(my_v[ip].y, j);
       }
     }
-    const int S = stamp_support(rec), base = stamp_base(rec);
-    if (S != Ladd.S) stamp_layout<G>(S, lane, W, Ladd);
-    const int S2 = S * S;
-    float2* tb = tile + base;
+    const int key = int(lane_u32(my_key, j));
+    const int S = key >> 16, base = key & 0xFFFF;
+    if (key != run) {
+      flush();
+      run = key;
+      if (S != Lrun.S) stamp_layout<G>(S, lane, W, Lrun);
+#pragma unroll
+      for (int r = 0; r < G; ++r)
+#pragma unroll
+        for (int ip = 0; ip < NP; ++ip) acc[r][ip] = T2{0, 0};
+    }
+    // this record's stamp into the run's sums (lanes past the stamp add values that their spare
+    // or masked cells discard)
 #pragma unroll
     for (int r = 0; r < G; ++r) {
-      if (64 * r >= S2) break;
-      if (Ladd.off[r] >= 0) {
-        const float wx = v[r].x, wy = CONJ ? -v[r].y : v[r].y;
-        float2* t = tb + Ladd.off[r];
+      const float wx = v[r].x, wy = CONJ ? -v[r].y : v[r].y;
 #pragma unroll
-        for (int ip = 0; ip < 4; ++ip) {
-          if (ip < npg) {
-            float2 x = t[ip * WW];
-            x.x = fmaf(vr[ip], wx, x.x);
-            x.x = fmaf(-vi[ip], wy, x.x);
-            x.y = fmaf(vr[ip], wy, x.y);
-            x.y = fmaf(vi[ip], wx, x.y);
-            t[ip * WW] = x;
-          }
-        }
-      }
+      for (int ip = 0; ip < NP; ++ip)
+        if (ip < npg) cmac(acc[r][ip], vr[ip], vi[ip], wx, wy);
     }
-    // stamps wider than 16 x 16: the cells past the first 256, without prefetch
+    // stamps wider than 16 x 16: the cells past the first 256 straight into the tile, without
+    // prefetch (distinct cells from the run's sums, which cover the first 256)
+    const int S2 = S * S;
     if (BIG && S2 > 256) {
-      const DDF_GLOBAL float2* cfp = (const DDF_GLOBAL float2*)(cf + (size_t(uint32_t(rec >> 32)) << 3));
+      T2* tb = tile + base;
+      const DDF_GLOBAL float2* cfp = (const DDF_GLOBAL float2*)(cf + o8);
       const float invS = __builtin_amdgcn_rcpf(float(S));
       for (int c = 256 + lane; c < S2; c += 64) {
         const int sy = int((float(c) + 0.5f) * invS);
         const float2 w = ld2(cfp + c);
         const float wx = w.x, wy = CONJ ? -w.y : w.y;
-        float2* t = tb + sy * W + (c - sy * S);
+        T2* t = tb + sy * W + (c - sy * S);
 #pragma unroll
-        for (int ip = 0; ip < 4; ++ip) {
+        for (int ip = 0; ip < NP; ++ip) {
           if (ip < npg) {
-            float2 x = t[ip * WW];
-            x.x = fmaf(vr[ip], wx, x.x);
-            x.x = fmaf(-vi[ip], wy, x.x);
-            x.y = fmaf(vr[ip], wy, x.y);
-            x.y = fmaf(vi[ip], wx, x.y);
+            T2 x = t[ip * WW];
+            cmac(x, vr[ip], vi[ip], wx, wy);
             t[ip * WW] = x;
           }
         }
       }
     }
   };
-  // Three records in flight, in statically rotating buffers (A, B, C), so no register copy has to
-  // wait for a load.
-  load(0, rA, vA);
-  load(1, rB, vB);
-  for (int j = 0; j < cnt; j += 3) {
-    load(j + 2, rC, vC);
-    add(j, rA, vA);
-    load(j + 3, rA, vA);
-    add(j + 1, rB, vB);
-    load(j + 4, rB, vB);
-    add(j + 2, rC, vC);
+  // D records in flight, in statically rotating buffers, so no register copy has to wait for a
+  // load.  A full batch (64 records) runs without index checks: the last D records are peeled off
+  // and issue only the one load still inside the batch.
+#pragma unroll
+  for (int s = 0; s < D - 1; ++s) load(s, oq[s], vq[s]);
+  if (FULL) {
+#pragma unroll 1
+    for (int j = 0; j < 64 - D; j += D) {
+#pragma unroll
+      for (int s = 0; s < D; ++s) {
+        load(j + s + D - 1, oq[(s + D - 1) % D], vq[(s + D - 1) % D]);
+        add(j + s, oq[s], vq[s]);
+      }
+    }
+    load(63, oq[D - 1], vq[D - 1]);
+#pragma unroll
+    for (int s = 0; s < D; ++s) add(64 - D + s, oq[s], vq[s]);
+  } else {
+    for (int j = 0; j < cnt; j += D) {
+#pragma unroll
+      for (int s = 0; s < D; ++s) {
+        load(j + s + D - 1, oq[(s + D - 1) % D], vq[(s + D - 1) % D]);
+        add(j + s, oq[s], vq[s]);
+      }
+    }
   }
+  flush();
 }
 
 // ONE: a single polarisation (npg == 1), with the per-polarisation loops resolved at compile time.
 // BIG: stamps may have more than 256 cells (halo > 7).
-template <bool CONJ, bool ONE, bool BIG>
+// WIDE: the tile sums in FP64 (the adjoint of degrid pass A in the FP64 precision, whose FP32 tile
+// sums would round at ~sqrt(records per cell) ulps of the result); tiles are flushed with atomics
+// then.
+template <bool CONJ, bool ONE, bool BIG, bool WIDE>
 __global__ __launch_bounds__(64) void grid_pass_b(BatchParams p, const uint64_t* bstamp, const float2* V,
                                                   const BinItem* items, const uint32_t* recs, uint32_t* queue,
                                                   float2* slots) {
-  // 16-byte aligned: the float2 read-modify-writes must not straddle 8-byte boundaries
+  using T2 = typename std::conditional<WIDE, double2, float2>::type;
+  // 16-byte aligned: the read-modify-writes must not straddle their own size
   extern __shared__ __attribute__((aligned(16))) float lds_f[];
-  float2* tile = reinterpret_cast<float2*>(lds_f);  // [npg][W * W]
+  T2* tile = reinterpret_cast<T2*>(lds_f);  // [npg][W * W], then 64 spare cells when npg == 1
   __shared__ uint32_t s_item;
   const int W = p.tile + 2 * p.halo;
   const int WW = W * W;
   const int npg = ONE ? 1 : (p.nstokes < p.npol ? p.nstokes : p.npol);
   const int lane = threadIdx.x;
+  int qx = int(blockIdx.x % kItemRanges), qleft = kItemRanges;
   while (true) {
-    if (lane == 0) s_item = atomicAdd(queue, 1u);
+    if (lane == 0) s_item = next_tile_item(queue, p.item_range, qx, qleft);
     __syncthreads();
     const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
     if (it >= p.nitems_b) break;
     const BinItem bi = items[it];
     const FacetDev fd = gload(p.facets + bi.facet);
-    const DDF_GLOBAL char* cf = gptr(reinterpret_cast<const char*>(fd.cf));
-    for (int i = lane; i < npg * WW; i += 64) tile[i] = make_float2(0.f, 0.f);
+    // the CF base in scalar registers (the loads address it with a 32-bit vector offset)
+    const uint64_t cfa = reinterpret_cast<uint64_t>(fd.cf);
+    const DDF_GLOBAL char* cf = (const DDF_GLOBAL char*)((uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(cfa >> 32)))
+                                                          << 32) |
+                                                         uint32_t(__builtin_amdgcn_readfirstlane(uint32_t(cfa))));
+    for (int i = lane; i < npg * WW; i += 64) tile[i] = T2{0, 0};
     const int x0 = bi.tx * p.tile - p.halo, y0 = bi.ty * p.tile - p.halo;
     for (uint32_t k0 = bi.begin; k0 < bi.end; k0 += 64u) {
       const uint32_t k = k0 + lane;
@@ -1024,36 +1140,49 @@ __global__ __launch_bounds__(64) void grid_pass_b(BatchParams p, const uint64_t*
 #pragma unroll
       for (int ip = 0; ip < 4; ++ip) my_v[ip] = (ok && ip < npg) ? V[my_v0 + ip] : make_float2(0.f, 0.f);
       const int cnt = int(min(64u, bi.end - k0));
-      // G = the largest ceil(S^2 / 64) of the batch, capped at 4
+      // each lane decodes its record once: the byte offsets of its sub-kernel and of the
+      // sub-kernel's last cell, and its run key (support << 16 | base)
       const int myS = stamp_support(my_rec);
+      const uint32_t my_o8 = uint32_t(my_rec >> 32) << 3;
+      const uint32_t my_lim8 = ok ? uint32_t(myS * myS - 1) << 3 : 0u;
+      const uint32_t my_key = uint32_t(my_rec & 0x1FFFFF);
+      // G = the largest ceil(S^2 / 64) of the batch, capped at 4
       const int myG = (myS * myS + 63) >> 6;
       const int G = __ballot(myG > 3) ? 4 : __ballot(myG > 2) ? 3 : __ballot(myG > 1) ? 2 : 1;
+      // (the full-batch form for one polarisation only: it doubles the code of a batch)
+#define DDF_GRID_B_BATCH(GG)                                                                                   \
+  (ONE && cnt == 64 ? grid_b_batch<CONJ, ONE, BIG, GG, true>(tile, W, WW, npg, lane, cf, my_o8, my_lim8, my_key, my_v, cnt) \
+             : grid_b_batch<CONJ, ONE, BIG, GG, false>(tile, W, WW, npg, lane, cf, my_o8, my_lim8, my_key, my_v, cnt))
       switch (G) {
-        case 1: grid_b_batch<CONJ, ONE, BIG, 1>(tile, W, WW, npg, lane, cf, my_rec, my_v, cnt); break;
-        case 2: grid_b_batch<CONJ, ONE, BIG, 2>(tile, W, WW, npg, lane, cf, my_rec, my_v, cnt); break;
-        case 3: grid_b_batch<CONJ, ONE, BIG, 3>(tile, W, WW, npg, lane, cf, my_rec, my_v, cnt); break;
-        default: grid_b_batch<CONJ, ONE, BIG, 4>(tile, W, WW, npg, lane, cf, my_rec, my_v, cnt); break;
+        case 1: DDF_GRID_B_BATCH(1); break;
+        case 2: DDF_GRID_B_BATCH(2); break;
+        case 3: DDF_GRID_B_BATCH(3); break;
+        default: DDF_GRID_B_BATCH(4); break;
       }
+#undef DDF_GRID_B_BATCH
     }
-    if (slots) {
+    if (!WIDE && slots) {
       // the whole tile into this item's slot with plain stores (grid_slot_reduce adds the slots)
       float4* dst = reinterpret_cast<float4*>(slots + size_t(bi.slot) * npg * WW);
       const float4* src = reinterpret_cast<const float4*>(tile);
       const int n4 = (npg * WW) / 2;
       for (int i = lane; i < n4; i += 64) dst[i] = src[i];
-      if ((npg * WW) & 1 && lane == 0) slots[size_t(bi.slot) * npg * WW + npg * WW - 1] = tile[npg * WW - 1];
+      if ((npg * WW) & 1 && lane == 0) {
+        const T2 x = tile[npg * WW - 1];
+        slots[size_t(bi.slot) * npg * WW + npg * WW - 1] = make_float2(float(x.x), float(x.y));
+      }
     } else {
       DDF_GLOBAL float* grid = gptr_w(reinterpret_cast<float*>(fd.grid));
       for (int i = lane; i < npg * WW; i += 64) {
         const int ip = i / WW;
         const int rem = i - ip * WW;
-        const float2 x = tile[i];
-        if (x.x == 0.f && x.y == 0.f) continue;
+        const T2 x = tile[i];
+        if (x.x == 0 && x.y == 0) continue;
         const int gy = y0 + rem / W, gx = x0 + rem % W;
         if (gx < 0 || gy < 0 || gx >= p.nx || gy >= p.ny) continue;
         const size_t gi = ((size_t(bi.band) * p.npol + ip) * p.ny + gy) * p.nx + gx;
-        atomic_add_g(grid + 2 * gi, x.x);
-        atomic_add_g(grid + 2 * gi + 1, x.y);
+        atomic_add_g(grid + 2 * gi, float(x.x));
+        atomic_add_g(grid + 2 * gi + 1, float(x.y));
       }
     }
     __syncthreads();
@@ -1122,8 +1251,9 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
   const int W = p.tile + 2 * p.halo;
   const int WW = W * W;
   const int ns = ONE ? 1 : p.nstokes;
+  int qx = int(blockIdx.x % kItemRanges), qleft = kItemRanges;
   while (true) {
-    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
+    if (threadIdx.x == 0) s_item = next_tile_item(queue, p.item_range, qx, qleft);
     __syncthreads();
     const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
     if (it >= p.nitems_b) break;
@@ -1610,7 +1740,7 @@ int batched_fill(const BatchParams& p, int64_t nblocks, const uint64_t* place, c
 // Records of every work item sorted by (CF plane, sub-kernel, record): one segmented radix sort
 // of 51-bit keys over the items' ranges (a plane needs 8 bits, a sub-kernel 11, a record 32).
 int batched_sort_items(const BatchParams& p, const uint64_t* place, uint32_t* recs, int64_t n, const BinItem* items,
-                       uint32_t nitems, hipStream_t st) {
+                       uint32_t nitems, bool stamp_order, hipStream_t st) {
   if (n <= 0 || nitems == 0) return DDF_OK;
   if (n >= (int64_t(1) << 31)) return DDF_EINVAL;
   uint64_t *k0 = nullptr, *k1 = nullptr;
@@ -1625,7 +1755,8 @@ int batched_sort_items(const BatchParams& p, const uint64_t* place, uint32_t* re
       hipMallocAsync(reinterpret_cast<void**>(&b1), size_t(nitems) * 4, st) != hipSuccess) {
     rc = DDF_ENOMEM;
   } else {
-    hipLaunchKernelGGL(plan_sort_keys, dim3(grid(n)), dim3(256), 0, st, p, place, recs, n, k0);
+    if (stamp_order) hipLaunchKernelGGL(plan_sort_keys<true>, dim3(grid(n)), dim3(256), 0, st, p, place, recs, n, k0);
+    else hipLaunchKernelGGL(plan_sort_keys<false>, dim3(grid(n)), dim3(256), 0, st, p, place, recs, n, k0);
     hipLaunchKernelGGL(plan_item_bounds, dim3(grid(nitems)), dim3(256), 0, st, items, nitems, b0, b1);
     if (hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, k0, k1, int(n), int(nitems), b0, b1, 0, 51, st) !=
             hipSuccess ||
@@ -1722,11 +1853,21 @@ static unsigned persistent_wgs(K kernel, size_t lds, unsigned nwg, int block = 2
 // grid_pass_b instance for a tile pass: CONJ, one polarisation or several, stamps beyond 256 cells.
 typedef void (*GridPassB)(BatchParams, const uint64_t*, const float2*, const BinItem*, const uint32_t*, uint32_t*,
                           float2*);
-template <bool CONJ>
+// LDS of a grid_pass_b workgroup: the tile image, plus 64 spare cells with one polarisation.
+static size_t grid_pass_b_lds(int npg, int W, bool wide) {
+  return (size_t(npg) * W * W + (npg == 1 ? 64 : 0)) * (wide ? sizeof(double2) : sizeof(float2));
+}
+// The adjoint of degrid pass A sums in FP64 in the FP64 precision, wherever those tiles fit 64 KB
+// of LDS.
+static bool adjoint_wide(int npg, int W, int prec) { return prec == 1 && grid_pass_b_lds(npg, W, true) <= 64 * 1024; }
+template <bool CONJ, bool WIDE>
 static GridPassB grid_pass_b_for(int npg, int halo) {
   const bool big = halo > 7;
-  if (npg == 1) return big ? grid_pass_b<CONJ, true, true> : grid_pass_b<CONJ, true, false>;
-  return big ? grid_pass_b<CONJ, false, true> : grid_pass_b<CONJ, false, false>;
+  if (npg == 1) return big ? grid_pass_b<CONJ, true, true, WIDE> : grid_pass_b<CONJ, true, false, WIDE>;
+  return big ? grid_pass_b<CONJ, false, true, WIDE> : grid_pass_b<CONJ, false, false, WIDE>;
+}
+static GridPassB grid_pass_b_adjoint_for(int npg, int halo, int W, int prec) {
+  return adjoint_wide(npg, W, prec) ? grid_pass_b_for<true, true>(npg, halo) : grid_pass_b_for<true, false>(npg, halo);
 }
 
 // degrid_pass_a instance for a precision (0 mixed, 1 FP64, 2 FP32) and one or several Stokes planes.
@@ -1743,10 +1884,11 @@ int batched_tile_wgs(const BatchParams& pg, const BatchParams& pd, unsigned out[
   const int Wg = pg.tile + 2 * pg.halo, Wd = pd.tile + 2 * pd.halo;
   const int npg = pg.nstokes < pg.npol ? pg.nstokes : pg.npol;
   const int npd = pd.nstokes < pd.npol ? pd.nstokes : pd.npol;
-  out[0] = persistent_wgs(grid_pass_b_for<false>(npg, pg.halo), size_t(npg) * Wg * Wg * sizeof(float2), 0, 64);
+  out[0] = persistent_wgs(grid_pass_b_for<false, false>(npg, pg.halo), grid_pass_b_lds(npg, Wg, false), 0, 64);
   const size_t lds_d = size_t(pd.nstokes) * Wd * Wd * 2 * sizeof(float);
   out[1] = persistent_wgs(degrid_pass_a_for(pd.prec, pd.nstokes == 1), lds_d, 0);
-  out[2] = persistent_wgs(grid_pass_b_for<true>(npd, pd.halo), size_t(pd.nstokes) * Wd * Wd * sizeof(float2), 0, 64);
+  out[2] = persistent_wgs(grid_pass_b_adjoint_for(npd, pd.halo, Wd, pd.prec),
+                          grid_pass_b_lds(npd, Wd, adjoint_wide(npd, Wd, pd.prec)), 0, 64);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
@@ -1755,8 +1897,8 @@ int batched_grid_b(const BatchParams& p, const uint64_t* place, const float2* V,
                    const uint32_t* tile_count, hipStream_t st) {
   const int W = p.tile + 2 * p.halo;
   const int npg = p.nstokes < p.npol ? p.nstokes : p.npol;
-  size_t lds = size_t(npg) * W * W * sizeof(float2);
-  hipLaunchKernelGGL(grid_pass_b_for<false>(npg, p.halo), dim3(nwg), dim3(64), lds, st, p, place, V, items, recs,
+  const size_t lds = grid_pass_b_lds(npg, W, false);
+  hipLaunchKernelGGL((grid_pass_b_for<false, false>(npg, p.halo)), dim3(nwg), dim3(64), lds, st, p, place, V, items, recs,
                      queue, reinterpret_cast<float2*>(slots));
   if (hipGetLastError() != hipSuccess) return DDF_EHIP;
   if (slots) {
@@ -1774,11 +1916,11 @@ int batched_grid_b(const BatchParams& p, const uint64_t* place, const float2* V,
 int batched_degrid_a_adjoint(const BatchParams& p, const uint64_t* place, const float2* Mbar, const BinItem* items,
                              const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st) {
   const int W = p.tile + 2 * p.halo;
-  size_t lds = size_t(p.nstokes) * W * W * sizeof(float2);
   BatchParams q = p;
   q.npol = p.npol;
   const int npg = q.nstokes < q.npol ? q.nstokes : q.npol;
-  hipLaunchKernelGGL(grid_pass_b_for<true>(npg, q.halo), dim3(nwg), dim3(64), lds, st, q, place, Mbar, items, recs,
+  const size_t lds = grid_pass_b_lds(npg, W, adjoint_wide(npg, W, q.prec));
+  hipLaunchKernelGGL(grid_pass_b_adjoint_for(npg, q.halo, W, q.prec), dim3(nwg), dim3(64), lds, st, q, place, Mbar, items, recs,
                      queue, static_cast<float2*>(nullptr));
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }

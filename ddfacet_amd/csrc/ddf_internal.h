@@ -165,6 +165,9 @@ struct FacetDev {
   double* sum_jones_chan;  // [2][nchan]
 };
 
+// Work-item ranges of a tile pass, one per XCD (workgroup b is dispatched to XCD b % 8).
+constexpr int kItemRanges = 8;
+
 struct BinItem {
   uint32_t begin, end;     // record range in the binned record list
   int16_t facet, band, tx, ty;
@@ -199,6 +202,9 @@ struct BatchParams {
   // tiles
   int tile, halo, ntiles_x, ntiles_y;
   uint32_t nitems_b;
+  // the work items split into kItemRanges contiguous ranges of about equal record counts:
+  // range x is items [item_range[x], item_range[x + 1])
+  const uint32_t* item_range;
   // blocks of the pass's BDA map: the per-(block, facet) values between the passes are stored
   // facet-major, [facet][block][nstokes]
   int64_t nblk;
@@ -218,7 +224,7 @@ int batched_count(const BatchParams& p, int64_t nblocks, const uint64_t* place, 
 int batched_fill(const BatchParams& p, int64_t nblocks, const uint64_t* place, const int64_t* bin_start,
                  uint32_t* fill, uint32_t* recs, hipStream_t st);
 int batched_sort_items(const BatchParams& p, const uint64_t* place, uint32_t* recs, int64_t n, const BinItem* items,
-                       uint32_t nitems, hipStream_t st);
+                       uint32_t nitems, bool stamp_order, hipStream_t st);
 int batched_vis_counts(const VisP& v, int32_t* cnt, hipStream_t st);
 int batched_vlist(const VisP& v, const int64_t* vis_start, uint64_t* vrc, uint32_t* vblk, hipStream_t st);
 int batched_reset(uint32_t* counters, int n, hipStream_t st);

@@ -1288,6 +1288,8 @@ struct ddf_plan {
   uint32_t* vrun = nullptr;       // run of each degrid list entry (block and Jones key shared)
   uint32_t* run_first = nullptr;  // first list entry of each run
   int64_t nruns = 0;
+  // work-queue counters: [0] grid pass A, [1, 9) grid pass B (one per item range), [9, 17) degrid
+  // pass A, [17] the adjoint of degrid pass B, [18, 26) the adjoint of degrid pass A
   uint32_t* queues = nullptr;
   const float2* jsrc[2] = {nullptr, nullptr};  // caller's DD / beam Jones tables
   double2* jprep[2] = {nullptr, nullptr};      // prepared copies (JonesP::pk / pb)
@@ -1404,8 +1406,8 @@ void fill_batch(BatchParams& p, const ddf_grid_desc* g, const ddf_facet* f0, con
 
 // Count -> scan -> fill of the valid records, then the work items (largest first).
 int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place, int chunk, uint32_t** recs_out,
-               BinItem** items_out, uint32_t* nitems, int64_t* nrec, hipStream_t st, uint32_t** tile_first_out = nullptr,
-               uint32_t** tile_count_out = nullptr) {
+               BinItem** items_out, uint32_t* nitems, int64_t* nrec, const uint32_t** ranges_out, bool stamp_order,
+               hipStream_t st, uint32_t** tile_first_out = nullptr, uint32_t** tile_count_out = nullptr) {
   const int64_t nsub = 2 * int64_t(p.nw);  // CF-plane sub-bins of each tile
   const int64_t ntile_all = int64_t(p.nf) * p.nband * p.ntiles_y * p.ntiles_x;
   const int64_t nbins = ntile_all * nsub;
@@ -1476,12 +1478,28 @@ int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place
     for (size_t k = 0; k < idx.size(); ++k) ordered[k] = items[idx[k]];
     items.swap(ordered);
   }
+  // kItemRanges contiguous ranges of the launch order with about equal record counts: the
+  // workgroups of one XCD take the items of their own range first, so the items in flight on an
+  // XCD come from few (facet, plane group)s and their CF planes stay in that XCD's L2
+  std::vector<uint32_t> range(kItemRanges + 1, uint32_t(items.size()));
+  {
+    range[0] = 0;
+    int64_t acc = 0;
+    int x = 1;
+    for (size_t k = 0; k < items.size() && x < kItemRanges; ++k) {
+      acc += int64_t(items[k].end) - items[k].begin;
+      while (x < kItemRanges && acc * kItemRanges >= tot * x) range[size_t(x++)] = uint32_t(k + 1);
+    }
+  }
   HIP_TRY(hipMemcpyAsync(bstart, hs.data(), size_t(nbins) * 8, hipMemcpyHostToDevice, st));
   uint32_t* recs = P->alloc<uint32_t>(size_t(tot));
   BinItem* d_items = P->alloc<BinItem>(items.size());
-  if (!recs || !d_items) return fail(DDF_ENOMEM, "plan allocation failed");
+  uint32_t* d_range = P->alloc<uint32_t>(range.size());
+  if (!recs || !d_items || !d_range) return fail(DDF_ENOMEM, "plan allocation failed");
   if (!items.empty())
     HIP_TRY(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(BinItem), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(d_range, range.data(), range.size() * 4, hipMemcpyHostToDevice, st));
+  *ranges_out = d_range;
   if (tile_first_out) {
     uint32_t* tf = P->alloc<uint32_t>(size_t(ntile_all));
     uint32_t* tc = P->alloc<uint32_t>(size_t(ntile_all));
@@ -1492,7 +1510,7 @@ int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place
     *tile_count_out = tc;
   }
   if (batched_fill(p, v.nblocks, place, bstart, fill, recs, st)) return fail(DDF_EHIP, "plan_fill launch failed");
-  if (batched_sort_items(p, place, recs, tot, d_items, uint32_t(items.size()), st))
+  if (batched_sort_items(p, place, recs, tot, d_items, uint32_t(items.size()), stamp_order, st))
     return fail(DDF_EHIP, "sorting the records of the work items failed");
   HIP_TRY(hipStreamSynchronize(st));
   *recs_out = recs;
@@ -1659,7 +1677,7 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
     d.sum_jones_chan = f.jones.sum_jones_chan;
   }
   P->d_facets = P->alloc<FacetDev>(hf.size());
-  P->queues = P->alloc<uint32_t>(4);
+  P->queues = P->alloc<uint32_t>(32);
   if (!P->d_facets || !P->queues) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
   if (hipMemcpyAsync(P->d_facets, hf.data(), hf.size() * sizeof(FacetDev), hipMemcpyHostToDevice, st) != hipSuccess)
     return bail(fail(DDF_EHIP, "plan upload failed"));
@@ -1723,8 +1741,8 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   // bins
   uint32_t ni = 0;
   int64_t nr = 0;
-  rc = build_bins(P, P->pg, P->vg, P->place_g, chunk, &P->recs_g, &P->items_g, &ni, &nr, st, &P->tile_first_g,
-                  &P->tile_count_g);
+  rc = build_bins(P, P->pg, P->vg, P->place_g, chunk, &P->recs_g, &P->items_g, &ni, &nr, &P->pg.item_range, true,
+                  st, &P->tile_first_g, &P->tile_count_g);
   if (rc) return bail(rc);
   P->pg.nitems_b = ni;
   P->stats[0] = nr;
@@ -1744,7 +1762,7 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
       if (!P->slots_g) (void)hipGetLastError();
     }
   }
-  rc = build_bins(P, P->pd, P->vd, P->place_d, chunk, &P->recs_d, &P->items_d, &ni, &nr, st);
+  rc = build_bins(P, P->pd, P->vd, P->place_d, chunk, &P->recs_d, &P->items_d, &ni, &nr, &P->pd.item_range, false, st);
   if (rc) return bail(rc);
   P->pd.nitems_b = ni;
   P->stats[1] = nr;
@@ -1854,7 +1872,7 @@ extern "C" int ddf_plan_grid(ddf_plan* P, float* vis, int32_t dopsf, void* strea
   BatchParams p = P->pg;
   p.dopsf = dopsf;
   if (prepare_jones(P, st)) return fail(DDF_EHIP, "Jones preparation failed");
-  if (batched_reset(P->queues, 2, st)) return fail(DDF_EHIP, "launch failed");
+  if (batched_reset(P->queues, 1 + kItemRanges, st)) return fail(DDF_EHIP, "launch failed");
   P->mark(DDF_PASS_GRID_A, 0, st);
   if (p.nitems_a && batched_grid_a(p, v, P->place_g, P->V, P->queues, P->nwg, st)) return fail(DDF_EHIP, "grid pass A launch failed");
   P->mark(DDF_PASS_GRID_A, 1, st);
@@ -1872,9 +1890,9 @@ extern "C" int ddf_plan_degrid(ddf_plan* P, float* vis, void* stream) {
   VisP v = P->vd;
   if (vis) v.vis = reinterpret_cast<float2*>(vis);
   if (prepare_jones(P, st)) return fail(DDF_EHIP, "Jones preparation failed");
-  if (batched_reset(P->queues + 2, 1, st)) return fail(DDF_EHIP, "launch failed");
+  if (batched_reset(P->queues + 1 + kItemRanges, kItemRanges, st)) return fail(DDF_EHIP, "launch failed");
   P->mark(DDF_PASS_DEGRID_A, 0, st);
-  if (P->pd.nitems_b && batched_degrid_a(P->pd, P->bstamp_d, P->M, P->items_d, P->recs_d, P->queues + 2, P->nwg_tiles[1], st))
+  if (P->pd.nitems_b && batched_degrid_a(P->pd, P->bstamp_d, P->M, P->items_d, P->recs_d, P->queues + 1 + kItemRanges, P->nwg_tiles[1], st))
     return fail(DDF_EHIP, "degrid pass A launch failed");
   P->mark(DDF_PASS_DEGRID_A, 1, st);
   P->mark(DDF_PASS_DEGRID_B, 0, st);
@@ -1888,14 +1906,14 @@ extern "C" int ddf_plan_degrid_adjoint(ddf_plan* P, const float* vis_bar, void* 
   if (!P || !vis_bar) return fail(DDF_EINVAL, "null argument");
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (prepare_jones(P, st)) return fail(DDF_EHIP, "Jones preparation failed");
-  if (batched_reset(P->queues + 2, 2, st)) return fail(DDF_EHIP, "launch failed");
+  if (batched_reset(P->queues + 1 + 2 * kItemRanges, 1 + kItemRanges, st)) return fail(DDF_EHIP, "launch failed");
   const float2* vb = reinterpret_cast<const float2*>(vis_bar);
   P->mark(DDF_PASS_ADJ_B, 0, st);
-  if (P->pd.nitems_a && batched_degrid_b_adjoint(P->pd, P->vd, P->place_d, vb, P->M, P->queues + 2, P->nwg, st))
+  if (P->pd.nitems_a && batched_degrid_b_adjoint(P->pd, P->vd, P->place_d, vb, P->M, P->queues + 1 + 2 * kItemRanges, P->nwg, st))
     return fail(DDF_EHIP, "degrid adjoint pass B launch failed");
   P->mark(DDF_PASS_ADJ_B, 1, st);
   P->mark(DDF_PASS_ADJ_A, 0, st);
-  if (P->pd.nitems_b && batched_degrid_a_adjoint(P->pd, P->bstamp_d, P->M, P->items_d, P->recs_d, P->queues + 3, P->nwg_tiles[2], st))
+  if (P->pd.nitems_b && batched_degrid_a_adjoint(P->pd, P->bstamp_d, P->M, P->items_d, P->recs_d, P->queues + 2 + 2 * kItemRanges, P->nwg_tiles[2], st))
     return fail(DDF_EHIP, "degrid adjoint pass A launch failed");
   P->mark(DDF_PASS_ADJ_A, 1, st);
   return DDF_OK;

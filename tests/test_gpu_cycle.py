@@ -120,12 +120,19 @@ def test_autograd_dot_product_adjoint(nband):
     rng = np.random.default_rng(11)
     x = torch.from_numpy(rng.standard_normal((nband, 1, geom.m, geom.m)).astype(np.float32)).cuda()
     y = torch.from_numpy((rng.standard_normal(P.vis.shape) + 1j * rng.standard_normal(P.vis.shape)).astype(np.complex64)).cuda()
-    ax = op.forward(x)
-    aty = op.adjoint(y)
-    lhs = torch.sum(torch.conj(ax).to(torch.complex128) * y.to(torch.complex128)).real.item()
-    rhs = torch.sum(x.double() * aty.double()).item()
-    print("dot-product test: <Ax,y>=%.8e <x,A^H y>=%.8e" % (lhs, rhs))
-    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+    ax = op.forward(x).to(torch.complex128)
+    y64 = y.to(torch.complex128)
+    lhs = torch.sum(torch.conj(ax) * y64).real.item()
+    rhs = torch.sum(x.double() * op.adjoint(y).double()).item()
+    # relative to the Cauchy-Schwarz scale |A x| |y| (random vectors: their inner product is far
+    # below it), then on y = A x, where <A x, A x> has no cancellation
+    scale = (torch.linalg.vector_norm(ax) * torch.linalg.vector_norm(y64)).item()
+    print("dot-product test: <Ax,y>=%.8e <x,A^H y>=%.8e |Ax||y|=%.4e rel %.2e" % (lhs, rhs, scale, abs(lhs - rhs) / scale))
+    assert abs(lhs - rhs) <= 1e-8 * scale
+    lhs2 = torch.sum(torch.conj(ax) * ax).real.item()
+    rhs2 = torch.sum(x.double() * op.adjoint(ax.to(torch.complex64)).double()).item()
+    print("coherent: <Ax,Ax>=%.12e <x,A^H Ax>=%.12e rel %.2e" % (lhs2, rhs2, abs(lhs2 - rhs2) / lhs2))
+    assert abs(lhs2 - rhs2) <= 1e-6 * lhs2
 
 
 def test_autograd_gradient_vs_finite_differences():
