@@ -1656,6 +1656,9 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
         const int64_t S = sup0[size_t(pl < nw ? pl : pl - nw)];
         total = std::max(total, off[size_t(pl)] + int64_t(ov) * ov * S * S);
       }
+      // grid pass B addresses a facet's CF values by a 32-bit byte offset
+      if (total * int64_t(sizeof(float2)) >= (int64_t(1) << 32))
+        return bail(fail(DDF_EINVAL, "a facet's CF table reaches 4 GiB"));
       float2* t = P->alloc<float2>(size_t(total));
       if (!t) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
       if (batched_cf_transpose(d.cf, t, f.cf.offset, P->d_support, nw, ov, maxsup, st))
