@@ -336,15 +336,16 @@ __global__ void reset_counter(uint32_t* c, int n) {
 }
 
 // Next work item of a tile pass for a workgroup, taken by one thread whose state (x, left) starts
-// at (blockIdx.x % kItemRanges, kItemRanges).  Workgroups are dispatched to the 8 XCDs round robin,
-// so the workgroups of one XCD take the items of their own range x (one counter per range) in
-// order; a workgroup whose range is used up moves on to the next ranges.  Returns 0xFFFFFFFF once
-// it has found every range used up (at most kItemRanges failed atomics per workgroup).
-__device__ __forceinline__ uint32_t next_tile_item(uint32_t* queue, const uint32_t* range, int& x, int& left) {
+// at (blockIdx.x % n, n) for n item ranges.  Workgroups are dispatched to the 8 XCDs round robin,
+// so with 8 ranges the workgroups of one XCD take the items of their own range x (one counter per
+// range) in order; a workgroup whose range is used up moves on to the next ranges.  Returns
+// 0xFFFFFFFF once it has found every range used up (at most n failed atomics per workgroup).
+__device__ __forceinline__ uint32_t next_tile_item(uint32_t* queue, const uint32_t* range, int n, int& x,
+                                                   int& left) {
   while (left > 0) {
     const uint32_t it = range[x] + atomicAdd(queue + x, 1u);
     if (it < range[x + 1]) return it;
-    x = x + 1 == kItemRanges ? 0 : x + 1;
+    x = x + 1 == n ? 0 : x + 1;
     --left;
   }
   return 0xFFFFFFFFu;
@@ -1114,9 +1115,9 @@ __global__ __launch_bounds__(64) void grid_pass_b(BatchParams p, const uint64_t*
   const int WW = W * W;
   const int npg = ONE ? 1 : (p.nstokes < p.npol ? p.nstokes : p.npol);
   const int lane = threadIdx.x;
-  int qx = int(blockIdx.x % kItemRanges), qleft = kItemRanges;
+  int qx = int(blockIdx.x % uint32_t(p.nranges)), qleft = p.nranges;
   while (true) {
-    if (lane == 0) s_item = next_tile_item(queue, p.item_range, qx, qleft);
+    if (lane == 0) s_item = next_tile_item(queue, p.item_range, p.nranges, qx, qleft);
     __syncthreads();
     const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
     if (it >= p.nitems_b) break;
@@ -1251,9 +1252,9 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
   const int W = p.tile + 2 * p.halo;
   const int WW = W * W;
   const int ns = ONE ? 1 : p.nstokes;
-  int qx = int(blockIdx.x % kItemRanges), qleft = kItemRanges;
+  int qx = int(blockIdx.x % uint32_t(p.nranges)), qleft = p.nranges;
   while (true) {
-    if (threadIdx.x == 0) s_item = next_tile_item(queue, p.item_range, qx, qleft);
+    if (threadIdx.x == 0) s_item = next_tile_item(queue, p.item_range, p.nranges, qx, qleft);
     __syncthreads();
     const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
     if (it >= p.nitems_b) break;

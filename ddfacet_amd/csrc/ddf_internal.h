@@ -165,8 +165,10 @@ struct FacetDev {
   double* sum_jones_chan;  // [2][nchan]
 };
 
-// Work-item ranges of a tile pass, one per XCD (workgroup b is dispatched to XCD b % 8).
+// Work-item ranges of a tile pass, one per XCD (workgroup b is dispatched to XCD b % 8), for plans
+// of at least kRangeMinItems work items.
 constexpr int kItemRanges = 8;
+constexpr size_t kRangeMinItems = 65536;
 
 struct BinItem {
   uint32_t begin, end;     // record range in the binned record list
@@ -202,9 +204,10 @@ struct BatchParams {
   // tiles
   int tile, halo, ntiles_x, ntiles_y;
   uint32_t nitems_b;
-  // the work items split into kItemRanges contiguous ranges of about equal record counts:
-  // range x is items [item_range[x], item_range[x + 1])
+  // the work items split into nranges (1 or kItemRanges) contiguous ranges of about equal record
+  // counts: range x is items [item_range[x], item_range[x + 1])
   const uint32_t* item_range;
+  int nranges;
   // blocks of the pass's BDA map: the per-(block, facet) values between the passes are stored
   // facet-major, [facet][block][nstokes]
   int64_t nblk;

@@ -1406,7 +1406,8 @@ void fill_batch(BatchParams& p, const ddf_grid_desc* g, const ddf_facet* f0, con
 
 // Count -> scan -> fill of the valid records, then the work items (largest first).
 int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place, int chunk, uint32_t** recs_out,
-               BinItem** items_out, uint32_t* nitems, int64_t* nrec, const uint32_t** ranges_out, bool stamp_order,
+               BinItem** items_out, uint32_t* nitems, int64_t* nrec, const uint32_t** ranges_out, int* nranges_out,
+               bool stamp_order,
                hipStream_t st, uint32_t** tile_first_out = nullptr, uint32_t** tile_count_out = nullptr) {
   const int64_t nsub = 2 * int64_t(p.nw);  // CF-plane sub-bins of each tile
   const int64_t ntile_all = int64_t(p.nf) * p.nband * p.ntiles_y * p.ntiles_x;
@@ -1480,15 +1481,18 @@ int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place
   }
   // kItemRanges contiguous ranges of the launch order with about equal record counts: the
   // workgroups of one XCD take the items of their own range first, so the items in flight on an
-  // XCD come from few (facet, plane group)s and their CF planes stay in that XCD's L2
-  std::vector<uint32_t> range(kItemRanges + 1, uint32_t(items.size()));
-  {
-    range[0] = 0;
+  // XCD come from few (facet, plane group)s and their CF planes stay in that XCD's L2.  Only with
+  // many items: with few, the heaviest items (neighbouring uv tiles) would share one XCD's CUs, so
+  // all items then form one range.
+  const int nranges = items.size() >= kRangeMinItems ? kItemRanges : 1;
+  std::vector<uint32_t> range(size_t(nranges) + 1, uint32_t(items.size()));
+  range[0] = 0;
+  if (nranges > 1) {
     int64_t acc = 0;
     int x = 1;
-    for (size_t k = 0; k < items.size() && x < kItemRanges; ++k) {
+    for (size_t k = 0; k < items.size() && x < nranges; ++k) {
       acc += int64_t(items[k].end) - items[k].begin;
-      while (x < kItemRanges && acc * kItemRanges >= tot * x) range[size_t(x++)] = uint32_t(k + 1);
+      while (x < nranges && acc * nranges >= tot * x) range[size_t(x++)] = uint32_t(k + 1);
     }
   }
   HIP_TRY(hipMemcpyAsync(bstart, hs.data(), size_t(nbins) * 8, hipMemcpyHostToDevice, st));
@@ -1500,6 +1504,7 @@ int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place
     HIP_TRY(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(BinItem), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(d_range, range.data(), range.size() * 4, hipMemcpyHostToDevice, st));
   *ranges_out = d_range;
+  *nranges_out = nranges;
   if (tile_first_out) {
     uint32_t* tf = P->alloc<uint32_t>(size_t(ntile_all));
     uint32_t* tc = P->alloc<uint32_t>(size_t(ntile_all));
@@ -1744,8 +1749,8 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
   // bins
   uint32_t ni = 0;
   int64_t nr = 0;
-  rc = build_bins(P, P->pg, P->vg, P->place_g, chunk, &P->recs_g, &P->items_g, &ni, &nr, &P->pg.item_range, true,
-                  st, &P->tile_first_g, &P->tile_count_g);
+  rc = build_bins(P, P->pg, P->vg, P->place_g, chunk, &P->recs_g, &P->items_g, &ni, &nr, &P->pg.item_range,
+                  &P->pg.nranges, true, st, &P->tile_first_g, &P->tile_count_g);
   if (rc) return bail(rc);
   P->pg.nitems_b = ni;
   P->stats[0] = nr;
@@ -1765,7 +1770,8 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
       if (!P->slots_g) (void)hipGetLastError();
     }
   }
-  rc = build_bins(P, P->pd, P->vd, P->place_d, chunk, &P->recs_d, &P->items_d, &ni, &nr, &P->pd.item_range, false, st);
+  rc = build_bins(P, P->pd, P->vd, P->place_d, chunk, &P->recs_d, &P->items_d, &ni, &nr, &P->pd.item_range,
+                  &P->pd.nranges, false, st);
   if (rc) return bail(rc);
   P->pd.nitems_b = ni;
   P->stats[1] = nr;
