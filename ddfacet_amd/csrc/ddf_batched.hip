@@ -151,6 +151,16 @@ __global__ __launch_bounds__(256) void plan_place(BatchParams p, const VisP v, u
   }
 }
 
+// Pair codes of a map: one byte per (block, facet), 0x80 | band for a placed record, 0 for a
+// dropped one; the cycle passes that need no more of a placement (grid pass A, the adjoint of
+// degrid pass B) read these instead of the 8-byte records.
+__global__ __launch_bounds__(256) void plan_pair_codes(const uint64_t* place, int64_t n, uint8_t* code) {
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+    const uint64_t rec = place[t];
+    code[t] = place_valid(rec) ? uint8_t(0x80u | uint32_t(unpack_place(rec).band)) : uint8_t(0);
+  }
+}
+
 // ---- plan: binning (counting sort) --------------------------------------------------------------
 // Bin = (facet, band, tile y, tile x, CF plane): the records of one tile are contiguous and, inside
 // the tile, grouped by CF plane so consecutive stamps reuse the same plane (cache locality).
@@ -408,7 +418,7 @@ __device__ __forceinline__ WalkOrder walk_order(const JonesP& jp, const BlockRec
 }
 
 template <int READ, int JMODE, int PREC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void grid_pass_a(BatchParams p, const VisP v, const uint64_t* place, float2* V,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void grid_pass_a(BatchParams p, const VisP v, const uint8_t* pcode, float2* V,
                                                    uint32_t* queue) {
   // LDS: per-facet sums of every facet [nf][nband*npol + 2*nband], flushed once at the end; the
   // per-channel Jones sums of the current facet group [G][2][nchan] (when they fit), flushed when
@@ -502,8 +512,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
     for (int64_t b = active ? next_block() : b1; active && b < b1; b = next_block()) {
       // every lane of the slot stages; lanes whose record is dropped (or past the last facet)
       // only stage
-      const uint64_t rec = place[b * p.nf + fc];
-      const bool doit = f < p.nf && place_valid(rec);
+      const uint32_t code = pcode[b * p.nf + fc];
+      const bool doit = f < p.nf && (code & 0x80u) != 0;
       const FacetDev fd = gload(p.facets + fc);
       const BlockDescRec bdr = read_block_desc(v, b);
       const BlockRec& br = bdr.br;
@@ -522,7 +532,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
       // block), in the arithmetic type of the run sums; otherwise over visibilities, in FP64
       using AT = typename std::conditional<JMODE == 1 && PREC != 1, float, double>::type;
       cx<AT> acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-      const Unpacked u = unpack_place(rec);
+      const int band = int(code & 0x7Fu);
       double BBblk = 0;
       JonesPair jblk;
       if (JMODE == 2 && doit) {
@@ -853,10 +863,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
         }
         double* a = s_acc + f * per_f;
         for (int ip = 0; ip < npg; ++ip) {
-          atomicAdd(&a[u.band * p.npol + ip], swt);
+          atomicAdd(&a[band * p.npol + ip], swt);
           if (JMODE != 0) {
-            atomicAdd(&a[nbp + u.band], sj);
-            atomicAdd(&a[nbp + p.nband + u.band], ssw);
+            atomicAdd(&a[nbp + band], sj);
+            atomicAdd(&a[nbp + p.nband + band], ssw);
           }
         }
       }
@@ -1616,7 +1626,7 @@ __global__ __launch_bounds__(256) void plan_run_ids(const int32_t* flag, const i
 // one (block, facet): Mbar = D^H (sum_k conj(corr_k) J0^H vbar_k J1), mode-2 Jones applied to the sum.
 // Lanes are facets; each lane walks its block's visibilities (all of them: degridding ignores flags).
 template <int NCORR, int JMODE, int PREC>
-__global__ __launch_bounds__(256) void degrid_adj_b(BatchParams p, const VisP v, const uint64_t* place,
+__global__ __launch_bounds__(256) void degrid_adj_b(BatchParams p, const VisP v, const uint8_t* pcode,
                                                     const float2* vbar, float2* Mbar, uint32_t* queue) {
   const int G = p.group;
   const int slots = p.slots;
@@ -1635,7 +1645,7 @@ __global__ __launch_bounds__(256) void degrid_adj_b(BatchParams p, const VisP v,
     const int64_t b0 = chunk * p.blocks_per_item;
     const int64_t b1 = min(b0 + p.blocks_per_item, v.nblocks);
     for (int64_t b = b0 + slot; active && f < p.nf && b < b1; b += slots) {
-      if (!place_valid(place[b * p.nf + f])) continue;
+      if (!(pcode[b * p.nf + f] & 0x80u)) continue;
       const FacetDev fd = gload(p.facets + f);
       BlockRec br = read_block(v, b);
       double fm = 0;
@@ -1716,6 +1726,14 @@ int batched_place(const BatchParams& p, const VisP& v, bool degrid, uint64_t* pl
   if (nb < 1) nb = 1;
   if (degrid) hipLaunchKernelGGL(plan_place<true>, dim3(unsigned(nb)), dim3(256), 0, st, p, v, place);
   else hipLaunchKernelGGL(plan_place<false>, dim3(unsigned(nb)), dim3(256), 0, st, p, v, place);
+  return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
+}
+
+int batched_pair_codes(const uint64_t* place, int64_t n, uint8_t* code, hipStream_t st) {
+  int64_t nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(plan_pair_codes, dim3(unsigned(nb)), dim3(256), 0, st, place, n, code);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
@@ -1800,28 +1818,28 @@ int batched_reset(uint32_t* counters, int n, hipStream_t st) {
 }
 
 template <int READ, int JMODE, int PREC>
-static void launch_a(const BatchParams& p, const VisP& v, const uint64_t* place, float2* V, uint32_t* q,
+static void launch_a(const BatchParams& p, const VisP& v, const uint8_t* pcode, float2* V, uint32_t* q,
                      size_t lds, unsigned nwg, hipStream_t st) {
-  hipLaunchKernelGGL((grid_pass_a<READ, JMODE, PREC>), dim3(nwg), dim3(256), lds, st, p, v, place, V, q);
+  hipLaunchKernelGGL((grid_pass_a<READ, JMODE, PREC>), dim3(nwg), dim3(256), lds, st, p, v, pcode, V, q);
 }
 
 template <int READ, int JMODE>
-static void launch_a_p(const BatchParams& p, const VisP& v, const uint64_t* place, float2* V, uint32_t* q,
+static void launch_a_p(const BatchParams& p, const VisP& v, const uint8_t* pcode, float2* V, uint32_t* q,
                        size_t lds, unsigned nwg, hipStream_t st) {
-  if (p.prec == 1) launch_a<READ, JMODE, 1>(p, v, place, V, q, lds, nwg, st);
-  else if (p.prec == 2) launch_a<READ, JMODE, 2>(p, v, place, V, q, lds, nwg, st);
-  else launch_a<READ, JMODE, 0>(p, v, place, V, q, lds, nwg, st);
+  if (p.prec == 1) launch_a<READ, JMODE, 1>(p, v, pcode, V, q, lds, nwg, st);
+  else if (p.prec == 2) launch_a<READ, JMODE, 2>(p, v, pcode, V, q, lds, nwg, st);
+  else launch_a<READ, JMODE, 0>(p, v, pcode, V, q, lds, nwg, st);
 }
 
 template <int READ>
-static void launch_a_j(const BatchParams& p, const VisP& v, const uint64_t* place, float2* V, uint32_t* q,
+static void launch_a_j(const BatchParams& p, const VisP& v, const uint8_t* pcode, float2* V, uint32_t* q,
                        size_t lds, unsigned nwg, hipStream_t st) {
   // per-visibility Jones: run-accumulated (1), or per visibility with ScaleJones (3), whose Jones
   // also depends on the uvw of the visibility that refreshed it
-  if (p.jmode == 1 && p.jp.scale) launch_a_p<READ, 3>(p, v, place, V, q, lds, nwg, st);
-  else if (p.jmode == 1) launch_a_p<READ, 1>(p, v, place, V, q, lds, nwg, st);
-  else if (p.jmode == 2) launch_a_p<READ, 2>(p, v, place, V, q, lds, nwg, st);
-  else launch_a_p<READ, 0>(p, v, place, V, q, lds, nwg, st);
+  if (p.jmode == 1 && p.jp.scale) launch_a_p<READ, 3>(p, v, pcode, V, q, lds, nwg, st);
+  else if (p.jmode == 1) launch_a_p<READ, 1>(p, v, pcode, V, q, lds, nwg, st);
+  else if (p.jmode == 2) launch_a_p<READ, 2>(p, v, pcode, V, q, lds, nwg, st);
+  else launch_a_p<READ, 0>(p, v, pcode, V, q, lds, nwg, st);
 }
 
 size_t batched_grid_a_stage_bytes(int group, int read) {
@@ -1830,13 +1848,13 @@ size_t batched_grid_a_stage_bytes(int group, int read) {
   return size_t(256 / 64) * size_t(64 / group) * size_t(group) * size_t(grid_a_stride(read)) * sizeof(float) + 16;
 }
 
-int batched_grid_a(const BatchParams& p, const VisP& v, const uint64_t* place, float2* V, uint32_t* queue,
+int batched_grid_a(const BatchParams& p, const VisP& v, const uint8_t* pcode, float2* V, uint32_t* queue,
                    unsigned nwg, hipStream_t st) {
   size_t lds = (size_t(p.nf) * (p.nband * p.npol + 2 * p.nband) + (p.chan_lds ? size_t(p.group) * (2 * v.nchan + 1) : 0)) * 8 +
                batched_grid_a_stage_bytes(p.group, p.read);
-  if (p.read == 4) launch_a_j<4>(p, v, place, V, queue, lds, nwg, st);
-  else if (p.read == 2) launch_a_j<2>(p, v, place, V, queue, lds, nwg, st);
-  else launch_a_j<1>(p, v, place, V, queue, lds, nwg, st);
+  if (p.read == 4) launch_a_j<4>(p, v, pcode, V, queue, lds, nwg, st);
+  else if (p.read == 2) launch_a_j<2>(p, v, pcode, V, queue, lds, nwg, st);
+  else launch_a_j<1>(p, v, pcode, V, queue, lds, nwg, st);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 
@@ -1983,29 +2001,29 @@ int batched_run_ids(const BatchParams& p, const VisP& v, const uint64_t* vrc, co
 namespace ddf {
 
 template <int NCORR, int JMODE>
-static void launch_adj_b_p(const BatchParams& p, const VisP& v, const uint64_t* place, const float2* vbar,
+static void launch_adj_b_p(const BatchParams& p, const VisP& v, const uint8_t* pcode, const float2* vbar,
                            float2* Mbar, uint32_t* q, unsigned nwg, hipStream_t st) {
   if (p.prec == 1)
-    hipLaunchKernelGGL((degrid_adj_b<NCORR, JMODE, 1>), dim3(nwg), dim3(256), 0, st, p, v, place, vbar, Mbar, q);
+    hipLaunchKernelGGL((degrid_adj_b<NCORR, JMODE, 1>), dim3(nwg), dim3(256), 0, st, p, v, pcode, vbar, Mbar, q);
   else if (p.prec == 2)
-    hipLaunchKernelGGL((degrid_adj_b<NCORR, JMODE, 2>), dim3(nwg), dim3(256), 0, st, p, v, place, vbar, Mbar, q);
+    hipLaunchKernelGGL((degrid_adj_b<NCORR, JMODE, 2>), dim3(nwg), dim3(256), 0, st, p, v, pcode, vbar, Mbar, q);
   else
-    hipLaunchKernelGGL((degrid_adj_b<NCORR, JMODE, 0>), dim3(nwg), dim3(256), 0, st, p, v, place, vbar, Mbar, q);
+    hipLaunchKernelGGL((degrid_adj_b<NCORR, JMODE, 0>), dim3(nwg), dim3(256), 0, st, p, v, pcode, vbar, Mbar, q);
 }
 
 template <int NCORR>
-static void launch_adj_b_j(const BatchParams& p, const VisP& v, const uint64_t* place, const float2* vbar,
+static void launch_adj_b_j(const BatchParams& p, const VisP& v, const uint8_t* pcode, const float2* vbar,
                            float2* Mbar, uint32_t* q, unsigned nwg, hipStream_t st) {
-  if (p.jmode == 1) launch_adj_b_p<NCORR, 1>(p, v, place, vbar, Mbar, q, nwg, st);
-  else if (p.jmode == 2) launch_adj_b_p<NCORR, 2>(p, v, place, vbar, Mbar, q, nwg, st);
-  else launch_adj_b_p<NCORR, 0>(p, v, place, vbar, Mbar, q, nwg, st);
+  if (p.jmode == 1) launch_adj_b_p<NCORR, 1>(p, v, pcode, vbar, Mbar, q, nwg, st);
+  else if (p.jmode == 2) launch_adj_b_p<NCORR, 2>(p, v, pcode, vbar, Mbar, q, nwg, st);
+  else launch_adj_b_p<NCORR, 0>(p, v, pcode, vbar, Mbar, q, nwg, st);
 }
 
-int batched_degrid_b_adjoint(const BatchParams& p, const VisP& v, const uint64_t* place, const float2* vbar,
+int batched_degrid_b_adjoint(const BatchParams& p, const VisP& v, const uint8_t* pcode, const float2* vbar,
                              float2* Mbar, uint32_t* queue, unsigned nwg, hipStream_t st) {
-  if (v.ncorr == 4) launch_adj_b_j<4>(p, v, place, vbar, Mbar, queue, nwg, st);
-  else if (v.ncorr == 2) launch_adj_b_j<2>(p, v, place, vbar, Mbar, queue, nwg, st);
-  else launch_adj_b_j<1>(p, v, place, vbar, Mbar, queue, nwg, st);
+  if (v.ncorr == 4) launch_adj_b_j<4>(p, v, pcode, vbar, Mbar, queue, nwg, st);
+  else if (v.ncorr == 2) launch_adj_b_j<2>(p, v, pcode, vbar, Mbar, queue, nwg, st);
+  else launch_adj_b_j<1>(p, v, pcode, vbar, Mbar, queue, nwg, st);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 

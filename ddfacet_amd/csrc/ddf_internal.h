@@ -222,6 +222,7 @@ constexpr int grid_a_stride(int read) { return (grid_a_key_offset(read) + 12 + 3
 
 // host launchers (ddf_batched.hip)
 int batched_place(const BatchParams& p, const VisP& v, bool degrid, uint64_t* place, hipStream_t st);
+int batched_pair_codes(const uint64_t* place, int64_t n, uint8_t* code, hipStream_t st);
 int batched_block_desc(const VisP& v, uint4* desc, hipStream_t st);
 int batched_count(const BatchParams& p, int64_t nblocks, const uint64_t* place, uint32_t* counts, hipStream_t st);
 int batched_fill(const BatchParams& p, int64_t nblocks, const uint64_t* place, const int64_t* bin_start,
@@ -233,7 +234,7 @@ int batched_vlist(const VisP& v, const int64_t* vis_start, uint64_t* vrc, uint32
 int batched_reset(uint32_t* counters, int n, hipStream_t st);
 // LDS bytes of the visibility staging buffers of gridding pass A (facet group size, corrs read)
 size_t batched_grid_a_stage_bytes(int group, int read);
-int batched_grid_a(const BatchParams& p, const VisP& v, const uint64_t* place, float2* V, uint32_t* queue,
+int batched_grid_a(const BatchParams& p, const VisP& v, const uint8_t* pcode, float2* V, uint32_t* queue,
                    unsigned nwg, hipStream_t st);
 // slots: null -> tiles added into the grids with global atomics; else one tile slot per work item
 // ([nitems][npg][2][W*W]) and grid_slot_reduce (tile_first / tile_count: the items of each tile)
@@ -250,7 +251,7 @@ int batched_degrid_b(const BatchParams& p, const VisP& v, const float2* M, const
 int batched_run_ids(const BatchParams& p, const VisP& v, const uint64_t* vrc, const uint32_t* vblk, int64_t nvl,
                     int32_t* flag, const int64_t* excl, uint32_t* vrun, uint32_t* run_first, int phase, hipStream_t st);
 
-int batched_degrid_b_adjoint(const BatchParams& p, const VisP& v, const uint64_t* place, const float2* vbar,
+int batched_degrid_b_adjoint(const BatchParams& p, const VisP& v, const uint8_t* pcode, const float2* vbar,
                              float2* Mbar, uint32_t* queue, unsigned nwg, hipStream_t st);
 int batched_degrid_a_adjoint(const BatchParams& p, const uint64_t* place, const float2* Mbar, const BinItem* items,
                              const uint32_t* recs, uint32_t* queue, unsigned nwg, hipStream_t st);

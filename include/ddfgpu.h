@@ -265,7 +265,7 @@ int ddf_plan_info(const ddf_plan* plan, int64_t* out, int32_t n);
 /* The plan's integer placement maps, a debug dump of what gridder.h:321-351 / degridder.h:170-200
  * compute per block (no reference entry point returns them; the reference's maps are observable
  * only through the cells a block touches): for the grid (degrid = 0) or degrid map, one 64-bit
- * record per (block, facet), block-major, copied on `stream` into the device buffer out (n >=
+ * record per (block, facet), block-major, computed on `stream` into the device buffer out (n >=
  * nblocks * nfacets): locx bits 0-15, locy 16-31, CF plane 32-39 (put order: Wplanes iw, then
  * WplanesConj nw + iw), oversampling sub-kernel io 40-47 and jo 48-55, band 56-62, valid bit 63
  * (0: block dropped, flagged, sparsified away or outside the band range).                     */
