@@ -361,6 +361,17 @@ __device__ __forceinline__ uint32_t next_tile_item(uint32_t* queue, const uint32
   return 0xFFFFFFFFu;
 }
 
+// a += (er + i ei) w: four FMAs in FP32; in FP64 the product then the sum, as the reference forms it
+__device__ __forceinline__ void cmac_run(cx<float>& a, float er, float ei, const cx<float>& w) {
+  a.re = fmaf(er, w.re, a.re);
+  a.re = fmaf(-ei, w.im, a.re);
+  a.im = fmaf(er, w.im, a.im);
+  a.im = fmaf(ei, w.re, a.im);
+}
+__device__ __forceinline__ void cmac_run(cx<double>& a, float er, float ei, const cx<double>& w) {
+  a = a + cx<double>{double(er), double(ei)} * w;
+}
+
 // ---- cycle: grid pass A --------------------------------------------------------------------------
 // One workgroup handles one facet group; lane = slot * G + facet-in-group.
 // JMODE: 0 no Jones, 1 per-visibility Jones (applied once per run of one Jones key), 2 per-block
@@ -637,12 +648,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
           const size_t vc = size_t(srow) * v.nchan + cc;
           const DDF_GLOBAL float2* vp = gptr(v.vis) + vc * v.ncorr;
           const DDF_GLOBAL double* q = gptr(v.uvw) + 3 * srow;
+          const float w = gptr(v.weights)[vc];
+          // the mixed precision stages the weighted values (w V, rounded once to FP32), so that
+          // the walk forms phasor x (w V) per facet without its own weight product
 #pragma unroll
           for (int k = 0; k < READ; ++k) {
-            e[kV + 2 * k] = vp[k].x;
-            e[kV + 2 * k + 1] = vp[k].y;
+            e[kV + 2 * k] = kScaled ? w * vp[k].x : vp[k].x;
+            e[kV + 2 * k + 1] = kScaled ? w * vp[k].y : vp[k].y;
           }
-          e[kW] = gptr(v.weights)[vc];
+          e[kW] = w;
           int fl = gptr(v.flags)[vc * v.ncorr] ? 1 : 0;
           e[kRow] = __int_as_float(int(srow));
           e[kCh] = __int_as_float(cc);
@@ -747,17 +761,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
               if (!p.dopsf) {
                 cx<RT> wt;
                 if (kScaled) {
+                  // the staged values carry the weight
                   const float2 pz = phasor_rev(su[0] * fd.l0 + su[1] * fd.m0 + su[2] * fd.n0);
-                  wt = cx<RT>{RT(pz.x * hd.x), RT(pz.y * hd.x)};
+                  wt = cx<RT>{RT(pz.x), RT(pz.y)};
                 } else {
                   wt = ph.template weighted<RT>(v, cv_, hd.x);
                 }
                 if (READ == 4) {
 #pragma unroll
-                  for (int q = 0; q < 4; ++q) racc[q] = racc[q] + cx<RT>{RT(ev[2 * q]), RT(ev[2 * q + 1])} * wt;
+                  for (int q = 0; q < 4; ++q) cmac_run(racc[q], ev[2 * q], ev[2 * q + 1], wt);
                 } else if (READ == 2) {
-                  racc[0] = racc[0] + cx<RT>{RT(ev[0]), RT(ev[1])} * wt;
-                  racc[3] = racc[3] + cx<RT>{RT(ev[2]), RT(ev[3])} * wt;
+                  cmac_run(racc[0], ev[0], ev[1], wt);
+                  cmac_run(racc[3], ev[2], ev[3], wt);
                 } else {
                   const cx<RT> x = cx<RT>{RT(ev[0]), RT(ev[1])} * wt;
                   racc[0] = racc[0] + x;
@@ -787,8 +802,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
               cd wt = cd{fw, 0};
               if (!p.dopsf) {
                 if (kScaled) {
+                  // the staged values carry the weight
                   const float2 pz = phasor_rev(su[0] * fd.l0 + su[1] * fd.m0 + su[2] * fd.n0);
-                  wt = cd{double(pz.x), double(pz.y)} * fw;
+                  wt = cd{double(pz.x), double(pz.y)};
                 } else {
                   wt = ph.at(v, cv_) * fw;
                 }
