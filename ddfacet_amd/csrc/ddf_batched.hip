@@ -171,24 +171,81 @@ __device__ __forceinline__ int64_t bin_of(const BatchParams& p, int64_t t, uint6
   return (((int64_t(f) * p.nband + u.band) * p.ntiles_y + ty) * p.ntiles_x + tx) * (2 * p.nw) + u.plane;
 }
 
+// Binning with one atomic per run of equal bins: a wavefront takes 64 consecutive blocks of one
+// facet (lane = block), whose placements mostly share a bin (the blocks of a baseline are
+// consecutive and move slowly across the uv tiles), and the lanes of one run of equal bins share one
+// counter update (plan_count) or one slot reservation (plan_fill).  The waves of one block range
+// read the same lines of the block-major placement array.
+struct BinRun {
+  int64_t bin;          // this lane's bin (-1: no valid record)
+  int64_t t;            // this lane's record
+  bool ok, head;        // valid record; first lane of its run
+  uint64_t valid, heads;  // masks of valid lanes and run heads
+};
+__device__ __forceinline__ BinRun bin_run(const BatchParams& p, int64_t nblocks, const uint64_t* place, int64_t w,
+                                          int lane) {
+  BinRun r;
+  const int f = int(w % p.nf);
+  const int64_t b = (w / p.nf) * 64 + lane;
+  r.t = b * p.nf + f;
+  r.ok = false;
+  r.bin = -1;
+  if (b < nblocks) {
+    const uint64_t rec = place[r.t];
+    if (place_valid(rec)) {
+      r.ok = true;
+      r.bin = bin_of(p, r.t, rec);
+    }
+  }
+  const int lo = __shfl_up(int(uint32_t(r.bin)), 1, 64), hi = __shfl_up(int(uint32_t(uint64_t(r.bin) >> 32)), 1, 64);
+  const int64_t pbin = int64_t((uint64_t(uint32_t(hi)) << 32) | uint32_t(lo));
+  // a valid lane after an invalid one (bin -1) always opens a run
+  r.head = r.ok && (lane == 0 || pbin != r.bin);
+  r.valid = __ballot(r.ok);
+  r.heads = __ballot(r.head);
+  return r;
+}
+
 __global__ __launch_bounds__(256) void plan_count(BatchParams p, int64_t nblocks, const uint64_t* place,
                                                   uint32_t* counts) {
-  const int64_t total = nblocks * p.nf;
-  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += int64_t(gridDim.x) * blockDim.x) {
-    uint64_t rec = place[t];
-    if (place_valid(rec)) atomicAdd(&counts[bin_of(p, t, rec)], 1u);
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = ((nblocks + 63) / 64) * p.nf;
+  const int64_t wstride = int64_t(gridDim.x) * (blockDim.x >> 6);
+  for (int64_t w = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); w < nwaves; w += wstride) {
+    const BinRun r = bin_run(p, nblocks, place, w, lane);
+    if (r.head) {
+      // the run: this lane up to the next head (lanes past a run's end are invalid)
+      const uint64_t above = r.heads & ~((2ull << lane) - 1);
+      const uint64_t upto = above ? (above & (~above + 1)) - 1 : ~0ull;
+      const uint64_t mine = upto & ~((1ull << lane) - 1);
+      atomicAdd(&counts[r.bin], uint32_t(__popcll(r.valid & mine)));
+    }
   }
 }
 
 __global__ __launch_bounds__(256) void plan_fill(BatchParams p, int64_t nblocks, const uint64_t* place,
                                                  const int64_t* bin_start, uint32_t* fill, uint32_t* recs) {
-  const int64_t total = nblocks * p.nf;
-  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += int64_t(gridDim.x) * blockDim.x) {
-    uint64_t rec = place[t];
-    if (!place_valid(rec)) continue;
-    int64_t bin = bin_of(p, t, rec);
-    uint32_t slot = atomicAdd(&fill[bin], 1u);
-    recs[bin_start[bin] + slot] = uint32_t(t);
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = ((nblocks + 63) / 64) * p.nf;
+  const int64_t wstride = int64_t(gridDim.x) * (blockDim.x >> 6);
+  for (int64_t w = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); w < nwaves; w += wstride) {
+    const BinRun r = bin_run(p, nblocks, place, w, lane);
+    uint32_t base = 0;
+    if (r.head) {
+      const uint64_t above = r.heads & ~((2ull << lane) - 1);
+      const uint64_t upto = above ? (above & (~above + 1)) - 1 : ~0ull;
+      const uint64_t mine = upto & ~((1ull << lane) - 1);
+      base = atomicAdd(&fill[r.bin], uint32_t(__popcll(r.valid & mine)));
+    }
+    // each valid lane: its run's head h (the highest head at or below it) and its rank in the run
+    const uint64_t below = r.heads & ((2ull << lane) - 1);
+    const int h = below ? 63 - __clzll(below) : 0;
+    const uint32_t hbase = uint32_t(__shfl(int(base), h, 64));
+    if (r.ok) {
+      const uint64_t between = ((1ull << lane) - 1) & ~((1ull << h) - 1);
+      const uint32_t slot = hbase + uint32_t(__popcll(r.valid & between));
+      recs[bin_start[r.bin] + slot] = uint32_t(r.t);
+    }
   }
 }
 
@@ -1754,8 +1811,9 @@ int batched_pair_codes(const uint64_t* place, int64_t n, uint8_t* code, hipStrea
 }
 
 int batched_count(const BatchParams& p, int64_t nblocks, const uint64_t* place, uint32_t* counts, hipStream_t st) {
-  int64_t total = nblocks * p.nf;
-  int64_t nb = (total + 255) / 256;
+  // four wavefronts of 64 blocks of one facet per workgroup
+  const int64_t nwaves = ((nblocks + 63) / 64) * p.nf;
+  int64_t nb = (nwaves + 3) / 4;
   if (nb > 65536) nb = 65536;
   if (nb < 1) nb = 1;
   hipLaunchKernelGGL(plan_count, dim3(unsigned(nb)), dim3(256), 0, st, p, nblocks, place, counts);
@@ -1764,8 +1822,8 @@ int batched_count(const BatchParams& p, int64_t nblocks, const uint64_t* place, 
 
 int batched_fill(const BatchParams& p, int64_t nblocks, const uint64_t* place, const int64_t* bin_start,
                  uint32_t* fill, uint32_t* recs, hipStream_t st) {
-  int64_t total = nblocks * p.nf;
-  int64_t nb = (total + 255) / 256;
+  const int64_t nwaves = ((nblocks + 63) / 64) * p.nf;
+  int64_t nb = (nwaves + 3) / 4;
   if (nb > 65536) nb = 65536;
   if (nb < 1) nb = 1;
   hipLaunchKernelGGL(plan_fill, dim3(unsigned(nb)), dim3(256), 0, st, p, nblocks, place, bin_start, fill, recs);
