@@ -249,6 +249,32 @@ __global__ __launch_bounds__(256) void plan_fill(BatchParams p, int64_t nblocks,
   }
 }
 
+// Record counts of every (tile, plane group): the sum of the group's plane bins (groups of
+// consecutive planes, the bounds g * nsub / ngrp), one thread per (tile, group).
+__global__ __launch_bounds__(256) void plan_group_counts(const uint32_t* counts, int64_t ntile, int nsub, int ngrp,
+                                                         uint32_t* gcount) {
+  for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < ntile * ngrp; k += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = k / ngrp;
+    const int g = int(k - i * ngrp);
+    uint32_t c = 0;
+    for (int q = g * nsub / ngrp; q < (g + 1) * nsub / ngrp; ++q) c += counts[i * nsub + q];
+    gcount[k] = c;
+  }
+}
+
+// First record of every bin: the tile's first record plus the counts of its earlier plane bins, one
+// thread per tile.
+__global__ __launch_bounds__(256) void plan_bin_starts(const uint32_t* counts, const int64_t* tile_start, int64_t ntile,
+                                                       int nsub, int64_t* bin_start) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < ntile; i += int64_t(gridDim.x) * blockDim.x) {
+    int64_t s = tile_start[i];
+    for (int q = 0; q < nsub; ++q) {
+      bin_start[i * nsub + q] = s;
+      s += counts[i * nsub + q];
+    }
+  }
+}
+
 // Order of the records inside a work item, independent of the atomics of plan_fill.  Degridding
 // (STAMP false): by CF plane, then oversampled sub-kernel, then record, so that the lanes of a
 // wavefront (consecutive records) read the same sub-kernels (one cache line for a cell instead of
@@ -1799,6 +1825,23 @@ int batched_place(const BatchParams& p, const VisP& v, bool degrid, uint64_t* pl
   if (nb < 1) nb = 1;
   if (degrid) hipLaunchKernelGGL(plan_place<true>, dim3(unsigned(nb)), dim3(256), 0, st, p, v, place);
   else hipLaunchKernelGGL(plan_place<false>, dim3(unsigned(nb)), dim3(256), 0, st, p, v, place);
+  return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
+}
+
+int batched_group_counts(const uint32_t* counts, int64_t ntile, int nsub, int ngrp, uint32_t* gcount, hipStream_t st) {
+  int64_t nb = (ntile * ngrp + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(plan_group_counts, dim3(unsigned(nb)), dim3(256), 0, st, counts, ntile, nsub, ngrp, gcount);
+  return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
+}
+
+int batched_bin_starts(const uint32_t* counts, const int64_t* tile_start, int64_t ntile, int nsub, int64_t* bin_start,
+                       hipStream_t st) {
+  int64_t nb = (ntile + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(plan_bin_starts, dim3(unsigned(nb)), dim3(256), 0, st, counts, tile_start, ntile, nsub, bin_start);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }
 

@@ -223,6 +223,9 @@ constexpr int grid_a_stride(int read) { return (grid_a_key_offset(read) + 12 + 3
 // host launchers (ddf_batched.hip)
 int batched_place(const BatchParams& p, const VisP& v, bool degrid, uint64_t* place, hipStream_t st);
 int batched_pair_codes(const uint64_t* place, int64_t n, uint8_t* code, hipStream_t st);
+int batched_group_counts(const uint32_t* counts, int64_t ntile, int nsub, int ngrp, uint32_t* gcount, hipStream_t st);
+int batched_bin_starts(const uint32_t* counts, const int64_t* tile_start, int64_t ntile, int nsub, int64_t* bin_start,
+                       hipStream_t st);
 int batched_block_desc(const VisP& v, uint4* desc, hipStream_t st);
 int batched_count(const BatchParams& p, int64_t nblocks, const uint64_t* place, uint32_t* counts, hipStream_t st);
 int batched_fill(const BatchParams& p, int64_t nblocks, const uint64_t* place, const int64_t* bin_start,

@@ -1432,70 +1432,70 @@ int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place
   uint32_t* counts = P->alloc<uint32_t>(size_t(nbins));
   uint32_t* fill = P->alloc<uint32_t>(size_t(nbins));
   int64_t* bstart = P->alloc<int64_t>(size_t(nbins));
-  if (!counts || !fill || !bstart) return fail(DDF_ENOMEM, "plan allocation failed");
-  HIP_TRY(hipMemsetAsync(counts, 0, size_t(nbins) * 4, st));
-  HIP_TRY(hipMemsetAsync(fill, 0, size_t(nbins) * 4, st));
-  if (batched_count(p, v.nblocks, place, counts, st)) return fail(DDF_EHIP, "plan_count launch failed");
-  std::vector<uint32_t> hc(static_cast<size_t>(nbins));
-  HIP_TRY(hipMemcpyAsync(hc.data(), counts, size_t(nbins) * 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  std::vector<int64_t> hs(static_cast<size_t>(nbins));
-  int64_t tot = 0;
-  std::vector<BinItem> items;
-  std::vector<uint32_t> tfirst(static_cast<size_t>(ntile_all)), tcount(static_cast<size_t>(ntile_all));
   // A tile's sub-bins (CF planes) are split into ngrp groups of consecutive planes, each chunked
   // into items.  Slots are numbered tile by tile (a tile's slots consecutive, for
   // grid_slot_reduce); the launch order is (facet, band, plane group, tile), so that the
   // workgroups running at one time read the CF planes of one group and those stay in L2.
   const int64_t ngrp = nsub < 4 ? nsub : 4;
-  std::vector<int64_t> order_key;
+  uint32_t* gcount = P->alloc<uint32_t>(size_t(ntile_all * ngrp));
+  int64_t* tstart = P->alloc<int64_t>(size_t(ntile_all));
+  if (!counts || !fill || !bstart || !gcount || !tstart) return fail(DDF_ENOMEM, "plan allocation failed");
+  HIP_TRY(hipMemsetAsync(counts, 0, size_t(nbins) * 4, st));
+  HIP_TRY(hipMemsetAsync(fill, 0, size_t(nbins) * 4, st));
+  if (batched_count(p, v.nblocks, place, counts, st)) return fail(DDF_EHIP, "plan_count launch failed");
+  // only the (tile, group) counts come to the host; the bin starts are formed on the device
+  if (batched_group_counts(counts, ntile_all, int(nsub), int(ngrp), gcount, st))
+    return fail(DDF_EHIP, "plan_group_counts launch failed");
+  std::vector<uint32_t> hg(static_cast<size_t>(ntile_all * ngrp));
+  HIP_TRY(hipMemcpyAsync(hg.data(), gcount, hg.size() * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  // tile-major pass: each tile's first record and first slot, each (tile, group)'s item count
+  std::vector<int64_t> ts(static_cast<size_t>(ntile_all));
+  std::vector<uint32_t> tfirst(static_cast<size_t>(ntile_all)), tcount(static_cast<size_t>(ntile_all));
+  int64_t tot = 0, nitem = 0;
   for (int64_t i = 0; i < ntile_all; ++i) {
-    tfirst[size_t(i)] = uint32_t(items.size());
-    int64_t c = 0;
-    for (int64_t q = 0; q < nsub; ++q) {
-      hs[size_t(i * nsub + q)] = tot + c;
-      c += hc[size_t(i * nsub + q)];
+    ts[size_t(i)] = tot;
+    tfirst[size_t(i)] = uint32_t(nitem);
+    for (int64_t g = 0; g < ngrp; ++g) {
+      const int64_t cg = hg[size_t(i * ngrp + g)];
+      tot += cg;
+      nitem += (cg + chunk - 1) / chunk;
     }
-    if (c) {
-      int64_t rem = i;
-      const int tx = int(rem % p.ntiles_x); rem /= p.ntiles_x;
-      const int ty = int(rem % p.ntiles_y); rem /= p.ntiles_y;
-      const int band = int(rem % p.nband); rem /= p.nband;
-      const int f = int(rem);
-      const int64_t tile_in_fb = int64_t(ty) * p.ntiles_x + tx;
-      int64_t g0 = 0;
-      for (int64_t g = 0; g < ngrp; ++g) {
-        const int64_t q1 = (g + 1) * nsub / ngrp;
-        int64_t cg = 0;
-        for (int64_t q = g * nsub / ngrp; q < q1; ++q) cg += hc[size_t(i * nsub + q)];
-        for (int64_t b = 0; b < cg; b += chunk) {
-          BinItem it;
-          it.begin = uint32_t(tot + g0 + b);
-          it.end = uint32_t(tot + g0 + (b + chunk < cg ? b + chunk : cg));
-          it.facet = int16_t(f);
-          it.band = int16_t(band);
-          it.tx = int16_t(tx);
-          it.ty = int16_t(ty);
-          it.slot = uint32_t(items.size());
-          items.push_back(it);
-          order_key.push_back(((int64_t(f) * p.nband + band) * ngrp + g) * (int64_t(p.ntiles_x) * p.ntiles_y) +
-                              tile_in_fb);
-        }
-        g0 += cg;
-      }
-    }
-    tcount[size_t(i)] = uint32_t(items.size()) - tfirst[size_t(i)];
-    tot += c;
+    tcount[size_t(i)] = uint32_t(nitem - tfirst[size_t(i)]);
   }
   if (tot >= (int64_t(1) << 32)) return fail(DDF_EINVAL, "too many (block, facet) records for one plan");
-  {
-    std::vector<size_t> idx(items.size());
-    for (size_t k = 0; k < idx.size(); ++k) idx[k] = k;
-    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return order_key[a] < order_key[b]; });
-    std::vector<BinItem> ordered(items.size());
-    for (size_t k = 0; k < idx.size(); ++k) ordered[k] = items[idx[k]];
-    items.swap(ordered);
+  // items in launch order
+  std::vector<BinItem> items;
+  items.reserve(size_t(nitem));
+  const int64_t ntile_fb = int64_t(p.ntiles_x) * p.ntiles_y;
+  for (int64_t fb = 0; fb < int64_t(p.nf) * p.nband; ++fb) {
+    for (int64_t g = 0; g < ngrp; ++g) {
+      for (int64_t tin = 0; tin < ntile_fb; ++tin) {
+        const int64_t i = fb * ntile_fb + tin;
+        int64_t g0 = 0, s0 = 0;  // records and items of the tile's earlier groups
+        for (int64_t h = 0; h < g; ++h) {
+          const int64_t ch = hg[size_t(i * ngrp + h)];
+          g0 += ch;
+          s0 += (ch + chunk - 1) / chunk;
+        }
+        const int64_t cg = hg[size_t(i * ngrp + g)];
+        for (int64_t b = 0, c = 0; b < cg; b += chunk, ++c) {
+          BinItem it;
+          it.begin = uint32_t(ts[size_t(i)] + g0 + b);
+          it.end = uint32_t(ts[size_t(i)] + g0 + (b + chunk < cg ? b + chunk : cg));
+          it.facet = int16_t(fb / p.nband);
+          it.band = int16_t(fb % p.nband);
+          it.tx = int16_t(tin % p.ntiles_x);
+          it.ty = int16_t(tin / p.ntiles_x);
+          it.slot = uint32_t(tfirst[size_t(i)] + s0 + c);
+          items.push_back(it);
+        }
+      }
+    }
   }
+  HIP_TRY(hipMemcpyAsync(tstart, ts.data(), ts.size() * 8, hipMemcpyHostToDevice, st));
+  if (batched_bin_starts(counts, tstart, ntile_all, int(nsub), bstart, st))
+    return fail(DDF_EHIP, "plan_bin_starts launch failed");
   // kItemRanges contiguous ranges of the launch order with about equal record counts: the
   // workgroups of one XCD take the items of their own range first, so the items in flight on an
   // XCD come from few (facet, plane group)s and their CF planes stay in that XCD's L2.  Only with
@@ -1512,7 +1512,6 @@ int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place
       while (x < nranges && acc * nranges >= tot * x) range[size_t(x++)] = uint32_t(k + 1);
     }
   }
-  HIP_TRY(hipMemcpyAsync(bstart, hs.data(), size_t(nbins) * 8, hipMemcpyHostToDevice, st));
   uint32_t* recs = P->alloc<uint32_t>(size_t(tot));
   BinItem* d_items = P->alloc<BinItem>(items.size());
   uint32_t* d_range = P->alloc<uint32_t>(range.size());
@@ -1535,6 +1534,12 @@ int build_bins(ddf_plan* P, BatchParams& p, const VisP& v, const uint64_t* place
   if (batched_sort_items(p, place, recs, tot, d_items, uint32_t(items.size()), stamp_order, st))
     return fail(DDF_EHIP, "sorting the records of the work items failed");
   HIP_TRY(hipStreamSynchronize(st));
+  // the binning buffers are not needed by the cycle
+  P->release(counts, size_t(nbins));
+  P->release(fill, size_t(nbins));
+  P->release(bstart, size_t(nbins));
+  P->release(gcount, size_t(ntile_all * ngrp));
+  P->release(tstart, size_t(ntile_all));
   *recs_out = recs;
   *items_out = d_items;
   *nitems = uint32_t(items.size());
