@@ -268,7 +268,9 @@ int ddf_plan_info(const ddf_plan* plan, int64_t* out, int32_t n);
  * record per (block, facet), block-major, computed on `stream` into the device buffer out (n >=
  * nblocks * nfacets): locx bits 0-15, locy 16-31, CF plane 32-39 (put order: Wplanes iw, then
  * WplanesConj nw + iw), oversampling sub-kernel io 40-47 and jo 48-55, band 56-62, valid bit 63
- * (0: block dropped, flagged, sparsified away or outside the band range).                     */
+ * (0: block dropped, flagged, sparsified away or outside the band range).  The records are
+ * recomputed from the chunk's arrays given to ddf_plan_create, which must still be valid (as for
+ * every call on the plan).                                                                    */
 int ddf_plan_placements(const ddf_plan* plan, int32_t degrid, uint64_t* out, int64_t n, void* stream);
 /* Per-pass timing, a measurement hook with no reference counterpart: with timing on, every pass
  * of ddf_plan_grid / ddf_plan_degrid / ddf_plan_degrid_adjoint is bracketed by HIP events on the
