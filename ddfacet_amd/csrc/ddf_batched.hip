@@ -697,6 +697,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
       ChanPhasor<PREC> ph;
       const int nch = br.ch1 - br.ch0;
       const int nvis = br.nr * nch;
+      // JMODE 1 run starts: the Jones key and validity (a row of the data, not flagged) of this
+      // lane's staged entry, and of the slot's last entry of the round before
+      int mk[6] = {0, 0, 0, 0, 0, 0}, lk[6] = {0, 0, 0, 0, 0, 0};
+      int mval = 0, lval = 0;
       for (int i0 = 0; i0 < nvis; i0 += K) {
         const int n = min(K, nvis - i0);
         // stage visibilities i0 .. i0 + n - 1 of the walk (values, weight, flag word, row, channel,
@@ -709,6 +713,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
           k[4] = gptr(jp.a0)[r];
           k[5] = gptr(jp.a1)[r];
         };
+        float* me = nullptr;  // this lane's staged entry (K = G: at most one per lane and round)
+        int mfl = 0;
+        mval = 0;
         for (int i = lf; i < n; i += G) {
           const int g = i0 + i;
           int rr, cc;
@@ -756,24 +763,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
           }
           if (JMODE == 1) {
             int* ek = reinterpret_cast<int*>(e + kKey);
-            int k[6];
-            key_of(srow, cc, k);
+            key_of(srow, cc, mk);
 #pragma unroll
-            for (int j = 0; j < 6; ++j) ek[j] = k[j];
-            bool opens = true;
-            if (g > 0) {
-              int pr, pc;
-              wo.at(g - 1, pr, pc);
-              const int64_t prow = bdr.row(pr);
-              if (prow >= 0 && prow < v.nrow && !gptr(v.flags)[(size_t(prow) * v.nchan + pc) * v.ncorr]) {
-                int kp[6];
-                key_of(prow, pc, kp);
-                opens = false;
-#pragma unroll
-                for (int j = 0; j < 6; ++j) opens = opens || kp[j] != k[j];
-              }
-            }
-            if (opens && !(kScaled && (fl & 1))) fl |= 2;
+            for (int j = 0; j < 6; ++j) ek[j] = mk[j];
+            mval = (fl & 1) ? 0 : 1;
+            me = e;
+            mfl = fl;
           }
           e[kF] = __int_as_float(fl);
           if (kScaled && (fl & 1)) {
@@ -782,6 +777,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
 #pragma unroll
             for (int k = 0; k < 2 * READ; ++k) e[kV + k] = 0.f;
           }
+        }
+        if (JMODE == 1) {
+          // An entry opens a run of one key unless its predecessor in the walk (the slot's previous
+          // lane, or for the first lane the last entry of the round before, kept by the slot's last
+          // lane) is a valid visibility of the same key; with the scaled uvw a flagged entry opens
+          // none.  The slot's lanes are all active here.
+          const int src = lf > 0 ? wl - 1 : sw * G + G - 1;
+          int pk[6];
+#pragma unroll
+          for (int j = 0; j < 6; ++j) {
+            const int a = __shfl(mk[j], src, 64), b = __shfl(lk[j], src, 64);
+            pk[j] = lf > 0 ? a : b;
+          }
+          const int pa = __shfl(mval, src, 64), pb = __shfl(lval, src, 64);
+          const int pval = lf > 0 ? pa : pb;
+          if (me) {
+            bool opens = true;
+            if (i0 + lf > 0 && pval) {
+              opens = false;
+#pragma unroll
+              for (int j = 0; j < 6; ++j) opens = opens || pk[j] != mk[j];
+            }
+            if (opens && !(kScaled && (mfl & 1))) me[kF] = __int_as_float(mfl | 2);
+          }
+#pragma unroll
+          for (int j = 0; j < 6; ++j) lk[j] = mk[j];
+          lval = mval;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
