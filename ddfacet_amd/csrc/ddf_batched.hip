@@ -1470,6 +1470,7 @@ struct RunInfo {
   int row, chan, mid;  // representative row and channel; middle row of the block (-1: none)
   uint32_t blk;
   double tscale;       // decorr_tscale of the mean frequency of the block's channels
+  JonesKey key;        // the Jones key of the run (complex64 Jones tables only)
 };
 
 template <int NCORR, int JMODE, int PREC>
@@ -1485,12 +1486,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
   __shared__ cx<RT> sY[kYCap];
   __shared__ RunInfo sRun[256];
   // l0, m0, n0 of the batch's facets: the visibility loop reads them from LDS, not from the facet
-  // table in global memory (whose loads the compiler cannot hoist or move to the scalar unit)
+  // table in global memory (whose loads the compiler cannot hoist or move to the scalar unit); the
+  // facets' DD Jones and beam directions, so that forming Y reads no facet record
   __shared__ double sGeo[kFMax][3];
+  __shared__ int2 sDir[kFMax];
   // mixed precision: each lane's pre-scaled uvw, read back after every batch's barrier so that it
   // is not held in registers (or spilled) across the forming of Y
   __shared__ double sU[3][256];
   const int t = threadIdx.x;
+  // with the complex64 Jones tables the run's Jones key is read with the run table, once per run,
+  // so that forming Y of a (run, facet) waits on the model value and the Jones entries only
+  const bool keyed = JMODE != 0 && jones_f32_path<RT>(p.jp);
   for (int64_t k0 = int64_t(blockIdx.x) * 256; k0 < nvl; k0 += int64_t(gridDim.x) * 256) {
     const int64_t k = k0 + t;
     const int64_t klast = min(k0 + 255, nvl - 1);
@@ -1504,6 +1510,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
       ri.mid = -1;
       ri.blk = 0;
       ri.tscale = 0;
+      ri.key = JonesKey{0, 0, 0, 0, 0, 0};
       const uint32_t kr = run_first[r0 + t];
       if (int64_t(kr) < nvl) {
         const uint64_t e = vrc[kr];
@@ -1526,6 +1533,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
             }
             if (JMODE == 2) ri.chan = (br.ch0 + br.ch1) / 2;
           }
+          const int jrow = JMODE == 2 ? ri.mid : ri.row;
+          if (keyed && jrow >= 0) ri.key = jones_key_f(p.jp, jrow, ri.chan);
         }
       }
       sRun[t] = ri;
@@ -1570,7 +1579,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
         sGeo[t][0] = fd.l0;
         sGeo[t][1] = fd.m0;
         sGeo[t][2] = fd.n0;
+        sDir[t] = make_int2(fd.kms_dir, fd.beam_dir);
       }
+      __syncthreads();
       // Y of every (run, facet of the batch); facets vary fastest over the threads, so the Jones
       // loads of one run's facets (directions) are contiguous in the prepared tables
       for (int j = t; j < nr * nfb; j += 256) {
@@ -1596,7 +1607,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
             }
           }
           if (nz) {
-            const FacetDev fd = gload(p.facets + f);
             // Stokes -> correlations and the Jones products in the precision's arithmetic type
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -1605,19 +1615,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
                 if (s < p.nstokes) cv.a[q] = cv.a[q] + cx<RT>{RT(p.sm[q][s][0]), RT(p.sm[q][s][1])} * cvt<RT>(st[s]);
             }
             JonesP jp = p.jp;
-            jp.kdir = fd.kms_dir;
-            jp.bdir = fd.beam_dir;
-            if (JMODE == 2 && ri.mid >= 0) {
+            const int2 dir = sDir[fi];
+            jp.kdir = dir.x;
+            jp.bdir = dir.y;
+            const int jrow = JMODE == 2 ? ri.mid : ri.row;
+            if (JMODE != 0 && jrow >= 0) {
               M2T<RT> J0, J1;
-              jones_pair_t<RT>(jp, ri.mid, ri.chan, v.uvw + 3 * int64_t(ri.mid), p.wl_mean, J0, J1);
-              cv = mul(mul(J0, cv), herm(J1));
-            }
-            if (JMODE == 1) {
-              M2T<RT> J0, J1;
-              jones_pair_t<RT>(jp, ri.row, ri.chan, v.uvw + 3 * int64_t(ri.row), p.wl_mean, J0, J1);
+              if (keyed) {
+                M2T<float> F0, F1;
+                jones_pair_f(jp, ri.key, F0, F1);
+                J0 = cvt<RT>(F0);
+                J1 = cvt<RT>(F1);
+              } else {
+                jones_pair_t<RT>(jp, jrow, ri.chan, v.uvw + 3 * int64_t(jrow), p.wl_mean, J0, J1);
+              }
               cv = mul(mul(J0, cv), herm(J1));
             }
             if (p.dp.enabled && ri.mid >= 0) {
+              const FacetDev fd = gload(p.facets + f);
               DecorrP dp = p.dp;
               dp.l0 = fd.dec_l0;
               dp.m0 = fd.dec_m0;

@@ -366,33 +366,66 @@ __device__ __forceinline__ M2T<float> prepared_jones_f(const float2* tab, const 
   return out;
 }
 
+// Whether jones_pair_t<T> reads the complex64 tables (jones_pair_f) rather than going through
+// jones_at: FP32 arithmetic, no ScaleJones (which depends on the visibility), a complex64 table.
+template <typename T>
+__device__ __forceinline__ bool jones_f32_path(const JonesP& jp) {
+  return !std::is_same<T, double>::value && !jp.scale && (jp.pkf || jp.pbf);
+}
+
+// The Jones key of (row, chan) for the complex64 tables: antennas, DD Jones time and channel, beam
+// time and channel (the indices of an absent table are not read).  It does not depend on the
+// direction, so a key read once serves every facet.
+struct JonesKey {
+  int a0, a1, kt, kc, bt, bc;
+};
+__device__ __forceinline__ JonesKey jones_key_f(const JonesP& jp, int64_t row, int chan) {
+  JonesKey k;
+  k.a0 = gptr(jp.a0)[row];
+  k.a1 = gptr(jp.a1)[row];
+  k.kt = k.kc = k.bt = k.bc = 0;
+  if (jp.pkf) {
+    k.kt = gptr(jp.ktime)[row];
+    k.kc = gptr(jp.kchan)[chan];
+  }
+  if (jp.pbf) {
+    k.bt = gptr(jp.btime)[row];
+    k.bc = gptr(jp.bchan)[chan];
+  }
+  return k;
+}
+
+// The Jones pair of a key from the complex64 tables in the directions jp.kdir / jp.bdir: DD Jones
+// times beam where both are given.
+__device__ __forceinline__ void jones_pair_f(const JonesP& jp, const JonesKey& k, M2T<float>& J0, M2T<float>& J1) {
+  if (jp.pkf) {
+    J0 = prepared_jones_f(jp.pkf, jp.kdims, k.kt, k.a0, jp.kdir, k.kc);
+    J1 = prepared_jones_f(jp.pkf, jp.kdims, k.kt, k.a1, jp.kdir, k.kc);
+    if (jp.pbf) {
+      J0 = mul(J0, prepared_jones_f(jp.pbf, jp.bdims, k.bt, k.a0, jp.bdir, k.bc));
+      J1 = mul(J1, prepared_jones_f(jp.pbf, jp.bdims, k.bt, k.a1, jp.bdir, k.bc));
+    }
+  } else {
+    J0 = prepared_jones_f(jp.pbf, jp.bdims, k.bt, k.a0, jp.bdir, k.bc);
+    J1 = prepared_jones_f(jp.pbf, jp.bdims, k.bt, k.a1, jp.bdir, k.bc);
+  }
+}
+
 // The Jones pair of (row, chan) in the arithmetic type T from the prepared tables: FP64 through
-// jones_at; FP32 from the complex64 copies (ScaleJones, which depends on the visibility, goes
-// through the FP64 path).
+// jones_at; FP32 from the complex64 copies where jones_f32_path<T> holds.
 template <typename T>
 __device__ __forceinline__ void jones_pair_t(const JonesP& jp, int64_t row, int chan, const double* uvw,
                                              double wl_mean, M2T<T>& J0, M2T<T>& J1) {
-  if (std::is_same<T, double>::value || jp.scale || !jp.pkf && !jp.pbf) {
+  if (!jones_f32_path<T>(jp)) {
     const JonesPair jj = jones_at<true>(jp, row, chan, uvw, wl_mean);
     J0 = cvt<T>(jj.J0);
     J1 = cvt<T>(jj.J1);
     return;
   }
-  const int a0 = gptr(jp.a0)[row], a1 = gptr(jp.a1)[row];
-  if (jp.pkf) {
-    const int t = gptr(jp.ktime)[row], c = gptr(jp.kchan)[chan];
-    J0 = cvt<T>(prepared_jones_f(jp.pkf, jp.kdims, t, a0, jp.kdir, c));
-    J1 = cvt<T>(prepared_jones_f(jp.pkf, jp.kdims, t, a1, jp.kdir, c));
-    if (jp.pbf) {
-      const int tb = gptr(jp.btime)[row], cb = gptr(jp.bchan)[chan];
-      J0 = mul(J0, cvt<T>(prepared_jones_f(jp.pbf, jp.bdims, tb, a0, jp.bdir, cb)));
-      J1 = mul(J1, cvt<T>(prepared_jones_f(jp.pbf, jp.bdims, tb, a1, jp.bdir, cb)));
-    }
-  } else {
-    const int tb = gptr(jp.btime)[row], cb = gptr(jp.bchan)[chan];
-    J0 = cvt<T>(prepared_jones_f(jp.pbf, jp.bdims, tb, a0, jp.bdir, cb));
-    J1 = cvt<T>(prepared_jones_f(jp.pbf, jp.bdims, tb, a1, jp.bdir, cb));
-  }
+  M2T<float> F0, F1;
+  jones_pair_f(jp, jones_key_f(jp, row, chan), F0, F1);
+  J0 = cvt<T>(F0);
+  J1 = cvt<T>(F1);
 }
 
 // Per-lane Jones cache with the refresh rule of JonesServer::updateJones (JonesServer.cc:193-298):
