@@ -334,16 +334,23 @@ __global__ __launch_bounds__(256) void plan_stamp_records(BatchParams p, const u
   }
 }
 
-__global__ __launch_bounds__(256) void plan_cf_transpose(const float2* in, float2* out, const int64_t* off,
-                                                         const int32_t* support, int nw, int overs) {
+// CF copy for degrid pass A: each sub-kernel's S x S cells with its rows padded to an even pitch
+// Sp = S + (S & 1) (the pad cell is zero), plane pl at twice its put-order offset (a padded plane,
+// O2 S Sp cells, fits below twice the next plane's offset), so that every row starts on a 16-byte
+// boundary and a lane reads two cells per load.
+__global__ __launch_bounds__(256) void plan_cf_pad(const float2* in, float2* out, const int64_t* off,
+                                                   const int32_t* support, int nw, int overs) {
   const int pl = blockIdx.y;
   const int S = support[pl < nw ? pl : pl - nw];
-  const int S2 = S * S, O2 = overs * overs;
+  const int Sp = S + (S & 1);
+  const int S2 = S * S, SSp = S * Sp, O2 = overs * overs;
   const int64_t o = off[pl];
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < int64_t(O2) * S2;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < int64_t(O2) * SSp;
        i += int64_t(gridDim.x) * blockDim.x) {
-    const int k = int(i / S2), c = int(i - int64_t(k) * S2);
-    out[o + int64_t(c) * O2 + k] = in[o + i];
+    const int k = int(i / SSp);
+    const int rem = int(i - int64_t(k) * SSp);
+    const int r = rem / Sp, c = rem - r * Sp;
+    out[2 * o + i] = c < S ? in[o + int64_t(k) * S2 + r * S + c] : make_float2(0.f, 0.f);
   }
 }
 
@@ -1366,7 +1373,9 @@ __global__ __launch_bounds__(256) void grid_slot_reduce(BatchParams p, const flo
 
 // ---- cycle: degrid pass A (LDS-staged gather) --------------------------------------------------
 // The model tile of a bin item is staged in LDS (complex64, 16-byte aligned); lanes are records:
-// each lane gathers its own record's S x S stamp, row by row, with FP32 products (grid x CF)
+// each lane gathers its own record's S x S stamp, row by row, from the padded CF copy (two cells
+// per 16-byte load: the pass is bound by the texture-address path of its gathers, so the number of
+// load instructions sets its time), with FP32 products (grid x CF)
 // summed in ACC (degridder.h:202-212 sums the FP32 products in double; ACC = double in the FP64
 // and mixed precisions, float in the FP32 precision).  No cross-lane work: the records of a bin
 // share their CF plane, so the lanes of a wavefront mostly share S and run in step.
@@ -1391,7 +1400,7 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
     if (it >= p.nitems_b) break;
     const BinItem bi = items[it];
     const FacetDev fd = gload(p.facets + bi.facet);
-    const DDF_GLOBAL float2* cft = gptr(fd.cft);
+    const DDF_GLOBAL float2* cfp = gptr(fd.cft);
     const int x0 = bi.tx * p.tile - p.halo, y0 = bi.ty * p.tile - p.halo;
     const DDF_GLOBAL float2* model = gptr(reinterpret_cast<const float2*>(fd.model));
     for (int i = threadIdx.x; i < ns * WW; i += blockDim.x) {
@@ -1409,20 +1418,27 @@ __global__ __launch_bounds__(256) void degrid_pass_a(BatchParams p, const uint64
       const uint64_t rec = bstamp[k];
       const uint32_t t = recs[k];
       const int S = int((rec >> 16) & 0x1F);
-      // cell c of sub-kernel `sub` at plane offset + c * overs^2 + sub of the transposed planes:
-      // the lanes of a wavefront (records of one plane) read neighbouring words
+      const int Sp = S + (S & 1);
+      // row r of sub-kernel `sub` at 2 x (plane offset) + sub S Sp + r Sp of the padded copy; the
+      // record holds the put-order offset plane offset + sub S^2
       const int sub = int((rec >> 21) & 0x7FF);
-      const DDF_GLOBAL float2* cp = cft + (int64_t(uint32_t(rec >> 32)) - int64_t(sub) * S * S + sub);
+      const int64_t poff = int64_t(uint32_t(rec >> 32)) - int64_t(sub) * S * S;
+      const DDF_GLOBAL v4f* cp = reinterpret_cast<const DDF_GLOBAL v4f*>(cfp + (2 * poff + int64_t(sub) * S * Sp));
       const int toff0 = int(rec & 0xFFFF);  // tile offset of the stamp's first cell
       ACC sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
       for (int r = 0; r < S; ++r) {
-        const DDF_GLOBAL float2* crow = cp + int64_t(r * S) * O2;
+        const DDF_GLOBAL v4f* crow = cp + (r * Sp >> 1);
         const float2* trow = tile + toff0 + r * W;
-        // four cells at a time, their CF loads issued together
+        // four cells at a time (two 16-byte loads), their CF loads issued together
         for (int c = 0; c < S; c += 4) {
           float2 w[4];
 #pragma unroll
-          for (int j = 0; j < 4; ++j) w[j] = c + j < S ? ld2(crow + int64_t(c + j) * O2) : make_float2(0.f, 0.f);
+          for (int h = 0; h < 2; ++h) {
+            v4f x = {0.f, 0.f, 0.f, 0.f};
+            if (c + 2 * h < S) x = crow[(c >> 1) + h];
+            w[2 * h] = make_float2(x.x, x.y);
+            w[2 * h + 1] = make_float2(x.z, x.w);
+          }
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             if (c + j >= S) break;
@@ -2185,13 +2201,13 @@ int batched_degrid_b_adjoint(const BatchParams& p, const VisP& v, const uint8_t*
 
 namespace ddf {
 
-int batched_cf_transpose(const float2* in, float2* out, const int64_t* off, const int32_t* support, int nw, int overs,
-                         int max_support, hipStream_t st) {
-  const int64_t n = int64_t(overs) * overs * max_support * max_support;
+int batched_cf_pad(const float2* in, float2* out, const int64_t* off, const int32_t* support, int nw, int overs,
+                   int max_support, hipStream_t st) {
+  const int64_t n = int64_t(overs) * overs * max_support * (max_support + 1);
   int64_t nb = (n + 255) / 256;
   if (nb > 1024) nb = 1024;
   if (nb < 1) nb = 1;
-  hipLaunchKernelGGL(plan_cf_transpose, dim3(unsigned(nb), unsigned(2 * nw)), dim3(256), 0, st, in, out, off, support, nw,
+  hipLaunchKernelGGL(plan_cf_pad, dim3(unsigned(nb), unsigned(2 * nw)), dim3(256), 0, st, in, out, off, support, nw,
                      overs);
   return hipGetLastError() == hipSuccess ? DDF_OK : DDF_EHIP;
 }

@@ -19,6 +19,9 @@ namespace ddf {
 // reverse).  Every buffer these kernels touch is device global memory; these views give the loads,
 // stores and atomics their global forms (gptr for reads, gptr_w for writes and atomics).
 #define DDF_GLOBAL __attribute__((address_space(1)))
+// four floats read or written as one 16-byte access (a native vector type, which copies across
+// address spaces)
+typedef float v4f __attribute__((ext_vector_type(4)));
 template <typename T>
 __device__ __forceinline__ const DDF_GLOBAL T* gptr(const T* p) {
   return (const DDF_GLOBAL T*)p;

@@ -155,7 +155,7 @@ struct FacetDev {
   void* grid;              // complex64 [nband][npol][ny][nx] (gridding output)
   const void* model;       // complex64 model grid (degridding input)
   const float2* cf;        // CF planes, put order (Wplanes then WplanesConj)
-  const float2* cft;       // the same planes transposed to [cell][sub-kernel] (degrid pass A)
+  const float2* cft;       // the same planes with rows padded to an even pitch, at twice the offsets (degrid pass A)
   const int64_t* cf_offset;// [2*nw]
   double cu, cv, l0, m0, n0;
   double dec_l0, dec_m0;   // LSmearing[5:7]
@@ -265,7 +265,7 @@ int batched_stamp_records(const BatchParams& p, const uint64_t* place, const uin
                           uint64_t* bstamp, hipStream_t st);
 // CF planes of one facet transposed: plane p (offset off[p], support support[p mod nw]) from
 // [sub-kernel][cell] to [cell][sub-kernel], at the same offsets
-int batched_cf_transpose(const float2* in, float2* out, const int64_t* off, const int32_t* support, int nw, int overs,
+int batched_cf_pad(const float2* in, float2* out, const int64_t* off, const int32_t* support, int nw, int overs,
                          int max_support, hipStream_t st);
 int batched_prepare_jones(const BatchParams& p, const float2* src, double2* dst, float2* dstf, int dd, hipStream_t st);
 

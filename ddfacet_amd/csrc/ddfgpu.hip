@@ -1672,24 +1672,26 @@ extern "C" int ddf_plan_create(const ddf_grid_desc* g, const ddf_vis* vgrid, con
     d.model = models[i];
     d.cf = reinterpret_cast<const float2*>(f.cf.data);
     {
-      // transposed copy of the facet's CF planes for degrid pass A (same offsets)
+      // padded copy of the facet's CF planes for degrid pass A: rows of an even pitch, plane pl at
+      // twice its offset
       const int nw = f.cf.nw, ov = f.cf.overs;
       std::vector<int64_t> off(size_t(2 * nw));
       if (hipMemcpyAsync(off.data(), f.cf.offset, off.size() * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
           hipStreamSynchronize(st) != hipSuccess)
         return bail(fail(DDF_EHIP, "reading the CF plane offsets failed"));
-      int64_t total = 0;
+      int64_t total = 0, total_pad = 0;
       for (int pl = 0; pl < 2 * nw; ++pl) {
         const int64_t S = sup0[size_t(pl < nw ? pl : pl - nw)];
         total = std::max(total, off[size_t(pl)] + int64_t(ov) * ov * S * S);
+        total_pad = std::max(total_pad, 2 * off[size_t(pl)] + int64_t(ov) * ov * S * (S + (S & 1)));
       }
       // grid pass B addresses a facet's CF values by a 32-bit byte offset
       if (total * int64_t(sizeof(float2)) >= (int64_t(1) << 32))
         return bail(fail(DDF_EINVAL, "a facet's CF table reaches 4 GiB"));
-      float2* t = P->alloc<float2>(size_t(total));
+      float2* t = P->alloc<float2>(size_t(total_pad));
       if (!t) return bail(fail(DDF_ENOMEM, "plan allocation failed"));
-      if (batched_cf_transpose(d.cf, t, f.cf.offset, P->d_support, nw, ov, maxsup, st))
-        return bail(fail(DDF_EHIP, "CF transpose launch failed"));
+      if (batched_cf_pad(d.cf, t, f.cf.offset, P->d_support, nw, ov, maxsup, st))
+        return bail(fail(DDF_EHIP, "CF copy launch failed"));
       d.cft = t;
     }
     d.cf_offset = f.cf.offset;
