@@ -1024,7 +1024,7 @@ __device__ __forceinline__ float lane_f32(float x, int j) { return __int_as_floa
 // Stamp record of a binned (block, facet) record (plan_stamp_records): base | S << 16 | sub << 21 |
 // cfoff << 32, with base the tile offset of the stamp's first cell (row (locy - sup) - y0, column
 // (locx - sup) - x0 of the (T + 2 h)^2 tile image), sub the oversampled sub-kernel index
-// io * OverS + jo (used with the transposed CF planes) and cfoff the element offset of the record's
+// io * OverS + jo (used with the padded CF copy of degrid pass A) and cfoff the element offset of the record's
 // oversampled S x S sub-kernel in its facet's CF buffer.  Cell c of the stamp (row c / S, column
 // c % S) sits at tile offset base + (c / S) W + c % S and takes CF value cf[cfoff + c].
 __device__ __forceinline__ int stamp_support(uint64_t srec) { return int((srec >> 16) & 0x1F); }

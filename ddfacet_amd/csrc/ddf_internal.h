@@ -263,10 +263,10 @@ int batched_tile_wgs(const BatchParams& pg, const BatchParams& pd, unsigned out[
 // stamp records (tile offset | support << 16 | CF offset << 32) of the binned records
 int batched_stamp_records(const BatchParams& p, const uint64_t* place, const uint32_t* recs, int64_t n,
                           uint64_t* bstamp, hipStream_t st);
-// CF planes of one facet transposed: plane p (offset off[p], support support[p mod nw]) from
-// [sub-kernel][cell] to [cell][sub-kernel], at the same offsets
+// CF planes of one facet copied for degrid pass A: plane p (offset off[p], support S =
+// support[p mod nw]) at offset 2 off[p], each sub-kernel's S rows padded to an even pitch with zeros
 int batched_cf_pad(const float2* in, float2* out, const int64_t* off, const int32_t* support, int nw, int overs,
-                         int max_support, hipStream_t st);
+                   int max_support, hipStream_t st);
 int batched_prepare_jones(const BatchParams& p, const float2* src, double2* dst, float2* dstf, int dd, hipStream_t st);
 
 }  // namespace ddf
